@@ -1,0 +1,244 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X SPH step engine.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1m|8m|16k|batch256|batch256-plain|batch64]
+
+One "step" is SimulationEngine::Update(dt, 1) over the whole workload: density,
+forces + integration + rehash, scan and scatter. The default workload is the
+single-scene dam break of 1,048,576 particles (128x64x128 jittered lattice at
+spacing h/2 in a 12.8x6.4x12.8 wall-bounded box, memory order shuffled, dt
+1e-3). Inputs are resident in HBM when the timed region starts. Prints one JSON
+line with particle-updates/s, the roofline of the dominant kernel (HIP events on
+the engine's stream over the timed region) and the CPU baseline (the restated
+reference CPU engine in oracle/, timed on a bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+METRIC = "particle-updates/s at 1M & 8M; batched particle-updates/s; % of HBM peak"
+
+CONFIGS = {
+    # name: (scene factory, dt, default steps)
+    "16k": ("cpu_16k", 1e-3, 200),
+    "1m": ("gpu_1m", 1e-3, 1000),
+    "8m": ("gpu_8m", 5e-4, 500),
+    "batch256": ("batch_256", 1e-3, 500),
+    "batch256-plain": ("batch_256", 1e-3, 100),
+    "batch64": ("batch_64_mixed", 1e-3, 200),
+}
+
+
+def stage_bytes(n: int, M: int, Ms: int, L: int):
+    """Algorithmic HBM bytes per launch of each stage (FP32 state, each compulsory
+    read or write counted once; neighbour gathers excluded)."""
+    return {
+        # pos r 12, p w 4; start table r; static table r
+        "density": 16 * n + 4 * (M + 1) + 4 * (Ms + 1),
+        # pos r 12, vel r 12, p r 4, pos w 12, vel w 12, key w 4; starts r + counts w; static r
+        "force": 56 * n + 8 * (M + 1) + 4 * (Ms + 1),
+        # counts r, starts w, next histogram cleared
+        "scan": 12 * L,
+        # key r 4, pos r/w 24, vel r/w 24, id r/w 8; cursors
+        "scatter": 60 * n + 4 * (M + 1),
+    }
+
+
+def step_bytes(n: int, M: int, Ms: int) -> int:
+    """B_alg per step of the measurement plan: 144 N + 24 (M + 1) + 8 (M_s + 1)."""
+    return 144 * n + 24 * (M + 1) + 8 * (Ms + 1)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def load_traffic(workload: str):
+    """HBM bytes per launch of the dominant kernel from a committed rocprofv3
+    --pmc summary (profiles/<kernel>_<workload>_pmc.md, line 'traffic_bytes: X'), if any."""
+    path = os.path.join(ROOT, "profiles", f"k_force_{workload}_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes:"):
+            try:
+                return float(line.split(":", 1)[1])
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(scene, dt: float, budget_s: float, threads: int):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    ref = orc.Oracle(scene.params, scene.pos, scene.vel, scene.static_pos, threads=threads)
+    t0 = time.perf_counter()
+    ref.update(dt, 1)  # warm-up step
+    t1 = time.perf_counter() - t0
+    steps = max(2, min(200, int(budget_s / max(t1, 1e-6))))
+    t0 = time.perf_counter()
+    ref.update(dt, steps)
+    el = time.perf_counter() - t0
+    ref.close()
+    return {
+        "value": scene.n * steps / el,
+        "unit": "particle-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{scene.name}: {steps} steps of {scene.n} particles after 1 warm-up step, "
+                  f"restated reference CPU engine (AoS, particleMap, serial scan, validators off), "
+                  f"{threads} threads, {el:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", default="1m", choices=sorted(CONFIGS))
+    ap.add_argument("--graph", action="store_true", help="replay steps from hipGraphs")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
+    ap.add_argument("--profile-all", action="store_true", help="time every stage with events (default: dominant)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is a design note, not an implementation)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    from sphfluidsimulation_amd import SimulationEngineHIP, scenes
+    from sphfluidsimulation_amd._lib import SPH_FLAG_HIP_GRAPH
+
+    factory, dt, default_steps = CONFIGS[args.config]
+    steps = args.steps if args.steps is not None else default_steps
+    t0 = time.perf_counter()
+    built = getattr(scenes, factory)()
+    scene_list = built if isinstance(built, list) else [built]
+    log(f"[bench] {args.config}: {len(scene_list)} scene(s), "
+        f"{sum(s.n for s in scene_list)} particles, generated in {time.perf_counter() - t0:.1f} s")
+
+    flags = SPH_FLAG_HIP_GRAPH if args.graph else 0
+    plain = args.config.endswith("-plain")
+    engines = []
+    if plain:
+        for sc in scene_list:
+            e = SimulationEngineHIP(0)
+            e.Initialize(sc, flags=flags)
+            engines.append(e)
+    else:
+        e = SimulationEngineHIP(0)
+        e.Initialize(scene_list if len(scene_list) > 1 else scene_list[0], flags=flags)
+        engines.append(e)
+    n_total = sum(e.n for e in engines)
+    M_total = sum(e.hash_map_size(s) for e in engines for s in range(e.n_scenes))
+    Ms_total = sum(sc.n_static for sc in scene_list)
+    L_total = sum(e.hash_map_size(s) + 1 for e in engines for s in range(e.n_scenes))
+    log(f"[bench] initialized: N={n_total} M={M_total} walls={Ms_total}")
+
+    def run(k):
+        for _ in range(k):
+            for e in engines:
+                e.Update(dt, 1)
+
+    run(args.warmup)
+    for e in engines:
+        e.synchronize()
+    log(f"[bench] warm-up {args.warmup} steps done")
+
+    # Timed region: events around the stages we report (the engine's own stream).
+    prof_stages = True if args.profile_all else ["force", "density"]
+    for e in engines:
+        e.set_profiling(prof_stages)
+    t0 = time.perf_counter()
+    run(steps)
+    for e in engines:
+        e.synchronize()
+    elapsed = time.perf_counter() - t0
+    times = {}
+    for e in engines:
+        for k, (ms, cnt) in e.stage_times().items():
+            a = times.setdefault(k, [0.0, 0])
+            a[0] += ms
+            a[1] += cnt
+        e.set_profiling(None)
+    log(f"[bench] timed {steps} steps in {elapsed:.3f} s; stage ms: "
+        + ", ".join(f"{k}={v[0]:.1f}/{v[1]}" for k, v in times.items()))
+
+    pu_s = n_total * steps / elapsed
+    ms_per_step = elapsed * 1e3 / steps
+    sb = stage_bytes(n_total, M_total, Ms_total, L_total)
+    # Dominant kernel: the timed stage with the largest total time.
+    timed = {k: v for k, v in times.items() if v[1] > 0}
+    dom = max(timed, key=lambda k: timed[k][0])
+    avg_s = timed[dom][0] / 1e3 / (timed[dom][1] / len(engines))
+    achieved = sb[dom] / avg_s / 1e9
+    workload = scene_list[0].name if len(scene_list) == 1 else f"{args.config}-{len(scene_list)}-scenes"
+    traffic = load_traffic(workload) if dom == "force" else None
+    b_step = step_bytes(n_total, M_total, Ms_total)
+    result = {
+        "metric": METRIC,
+        "value": pu_s,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": workload,
+            "particles": n_total,
+            "static_particles": Ms_total,
+            "scenes": len(scene_list),
+            "launch": "one engine per scene" if plain else "batched" if len(scene_list) > 1 else "single scene",
+            "dt": dt,
+            "hip_graph": bool(args.graph),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_" + dom,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "avg_launch_us": avg_s * 1e6,
+            "algorithmic_bytes_per_launch": sb[dom],
+        },
+        "step_roofline": {
+            "b_alg_per_step": b_step,
+            "achieved_GBs": b_step * steps / elapsed / 1e9,
+            "frac": b_step * steps / elapsed / 1e9 / HBM_PEAK_GBS,
+        },
+        "stage_ms_per_step": {k: v[0] / max(1, v[1] / len(engines)) for k, v in timed.items()},
+    }
+    for e in engines:
+        e.close()
+
+    if not args.no_cpu_baseline and len(scene_list) == 1:
+        threads = min(os.cpu_count() or 1, 16)
+        log(f"[bench] CPU baseline: {threads} threads, ~{args.cpu_budget:.0f} s")
+        result["cpu_baseline"] = cpu_baseline(scene_list[0], dt, args.cpu_budget, threads)
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

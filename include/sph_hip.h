@@ -1,0 +1,195 @@
+/*
+ * sph_hip.h — C ABI of the MI355X (gfx950) SPH step engine.
+ *
+ * This is the drop-in boundary for the step hot path of Murk42/SPHFluidSimulation:
+ * the reference's engines sit behind the C++ interface SPH::SimulationEngine
+ * (SPHFluidSimulation/include/SPH/Core/SimulationEngine.h:35-48) and exchange
+ * particle data through SPH::ParticleBufferManager
+ * (SPHFluidSimulation/include/SPH/Core/ParticleBufferManager.h:8-25). Every
+ * entry point below names the reference interface it replaces. The C++ adapter
+ * that implements SPH::SimulationEngine on top of these calls is
+ * include/sph/simulation_engine_hip.hpp.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative SPH_E_* code on error;
+ *     nothing throws across the ABI. sph_last_error() returns the message of the
+ *     last failure on a handle.
+ *   - One engine per handle; a handle is not thread-safe (the reference is driven
+ *     from one thread, the application's main loop).
+ *   - Device memory belongs to the handle. Pointers passed in are host pointers,
+ *     read during the call only.
+ *   - Positions and velocities are packed xyz float triples. "Original order" is
+ *     the order of the arrays given to sph_initialize / sph_batch_initialize.
+ */
+#ifndef SPH_HIP_H
+#define SPH_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define SPH_OK 0
+#define SPH_E_INVALID (-1)       /* bad handle or argument */
+#define SPH_E_HIP (-2)           /* a HIP runtime call failed */
+#define SPH_E_STATE (-3)         /* call not valid in the current state */
+#define SPH_E_VALIDATION (-4)    /* sph_validate found a broken invariant */
+#define SPH_E_NO_DEVICE (-5)     /* no HIP device */
+
+/* sph_initialize / sph_batch_initialize flags. */
+#define SPH_FLAG_HIP_GRAPH 0x1u     /* replay each step from a captured hipGraph */
+#define SPH_FLAG_RECORD_DEBUG 0x2u  /* keep per-particle density and acceleration of the last step */
+#define SPH_FLAG_FP16_OFFSETS 0x4u  /* mixed precision: FP16 cell-centre offsets, Kahan density sums */
+
+/* sph_validate result bits (the invariants of SimulationEngine.h:115-282). */
+#define SPH_V_NONFINITE 0x1u        /* NaN or Inf in position, velocity or pressure */
+#define SPH_V_KEY 0x2u              /* stored key != key recomputed from position */
+#define SPH_V_TABLE_FIRST 0x4u      /* table[0] != 0 */
+#define SPH_V_TABLE_LAST 0x8u       /* table[M] != N */
+#define SPH_V_TABLE_ORDER 0x10u     /* table not monotone or above N */
+#define SPH_V_BUCKET 0x20u          /* a particle lies outside its bucket's range */
+#define SPH_V_STATIC 0x40u          /* one of the above on the static (wall) table */
+
+/* Stages reported by sph_stage_times, in launch order. */
+#define SPH_STAGE_DENSITY 0   /* UpdateParticlePressure, SPHFunctions.cpp:358-501 */
+#define SPH_STAGE_FORCE 1     /* UpdateParticleDynamics + hash + histogram, SPHFunctions.cpp:503-656, 321-331 */
+#define SPH_STAGE_SCAN 2      /* prefix sum, SimulationEngineCPU.cpp:245-253 */
+#define SPH_STAGE_SCATTER 3   /* bucket fill / reorder, SPHFunctions.cpp:332-356 */
+#define SPH_STAGE_COUNT 4
+
+/* ParticleBehaviourParameters (Particles.h:23-41 / SPHFunctions.cpp:23-41),
+ * same 11-float layout. selfDensity and smoothingKernelConstant are outputs:
+ * the engine derives them as SimulationEngineCPU.cpp:406-407 does and ignores
+ * the values passed in. elasticity is carried but unused, as in the reference. */
+typedef struct sph_params {
+    float particleMass;
+    float gasConstant;
+    float elasticity;
+    float viscosity;
+    float gravityX;
+    float gravityY;
+    float gravityZ;
+    float restDensity;
+    float maxInteractionDistance;
+    float selfDensity;
+    float smoothingKernelConstant;
+} sph_params;
+
+typedef struct sph_engine* sph_handle;
+
+/* Constructor of SimulationEngineGPU (SimulationEngineGPU.cpp:32) /
+ * SimulationEngineCPU (SimulationEngineCPU.cpp:351): binds the handle to a HIP
+ * device and creates its stream. */
+int sph_create(sph_handle* out, int device);
+
+/* Destructor (SimulationEngineCPU.cpp:361-364): Clear() and free the handle. */
+int sph_destroy(sph_handle h);
+
+/* SimulationEngine::SystemImplementationName (SimulationEngine.h:44):
+ * "CPU" / "GPU" in the reference, "HIP" here. */
+const char* sph_system_implementation_name(void);
+
+/* SimulationEngine::Initialize (SimulationEngine.h:41; SimulationEngineCPU.cpp:396-504):
+ * copies one scene's dynamic particles (n, with velocities; vel may be NULL for
+ * zero) and static wall particles (n_s, may be 0) to the device and sorts both
+ * into buckets, M = 2n dynamic buckets and M_s = n_s static buckets
+ * (SimulationEngineCPU.cpp:468, 486). Within a bucket particles are ordered by
+ * descending original index, the order a one-thread reference run produces
+ * (SimulationEngineCPU.cpp:55-131, SimulationEngine.h:79-100). */
+int sph_initialize(sph_handle h, const sph_params* params, const float* dyn_pos, const float* dyn_vel, uint64_t n,
+                   const float* static_pos, uint64_t n_s, uint32_t flags);
+
+/* Batched Initialize (no reference counterpart; the reference runs one scene per
+ * engine): n_scenes independent scenes advance together, one launch per stage.
+ * dyn_offsets and static_offsets hold n_scenes + 1 prefix offsets into the
+ * concatenated dyn_pos/dyn_vel and static_pos arrays; params holds one
+ * sph_params per scene. Each scene keeps its own table of 2*N_s + 1 entries. */
+int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* params, const uint64_t* dyn_offsets,
+                         const float* dyn_pos, const float* dyn_vel, const uint64_t* static_offsets,
+                         const float* static_pos, uint32_t flags);
+
+/* Triangle boundary mesh of one scene (SceneBlueprint::GetMesh, SceneBlueprint.h:20;
+ * SimulationEngineCPU.cpp:418): count triangles of 9 floats (p1, p2, p3). */
+int sph_set_triangles(sph_handle h, uint32_t scene, const float* tri9, uint64_t count);
+
+/* SimulationEngine::Update (SimulationEngine.h:42; SimulationEngineCPU.cpp:420-459):
+ * enqueues `steps` steps of size dt on the engine's stream and returns. Unlike
+ * the reference's CPU engine it never drops an update. Update before Initialize
+ * returns SPH_E_STATE; n == 0 returns SPH_OK and does nothing
+ * (SimulationEngineCPU.cpp:422-429). */
+int sph_update(sph_handle h, float dt, uint32_t steps);
+
+/* Waits for all enqueued work (ParticleBufferManager::FlushAllOperations,
+ * ParticleBufferManager.h:24). */
+int sph_synchronize(sph_handle h);
+
+/* ParticleBufferManager::LockRead of the current buffer
+ * (OfflineCPUParticleBufferManager.cpp:61-67, RenderableGPUParticleBufferManager.cpp:256-270),
+ * returned in original order rather than the reference's memory order.
+ * pos and vel take 3 floats per particle, pressure 1; any may be NULL. */
+int sph_read(sph_handle h, float* pos, float* vel, float* pressure);
+
+/* Stored bucket key per particle (DynamicParticle::hash, Particles.h:16 /
+ * velocityAndHash.w, SPHFunctions.cpp:328, 645), in original order; scene-local. */
+int sph_read_keys(sph_handle h, uint32_t* keys);
+
+/* The bucket-start table (the reference's hashMap, SimulationEngineCPU.cpp:486-487):
+ * per scene M + 1 scene-local entries, concatenated over scenes. */
+int sph_read_cell_starts(sph_handle h, uint32_t* starts);
+
+/* Particles in bucket order (what the reference's particleMap indexes,
+ * SPHFunctions.cpp:346-356), as original (global) indices. */
+int sph_read_bucket_ids(sph_handle h, uint32_t* ids);
+
+/* Static table (per scene M_s + 1 scene-local entries) and the original
+ * (global) wall index of each sorted wall particle; either may be NULL. */
+int sph_read_static(sph_handle h, uint32_t* starts, uint32_t* ids);
+
+/* Per-particle density and acceleration (3 floats) of the last step, original
+ * order; needs SPH_FLAG_RECORD_DEBUG. */
+int sph_read_debug(sph_handle h, float* density, float* accel);
+
+/* FP64 conserved quantities per scene: out[8*s + {0..7}] =
+ * {mass, momentum x, y, z, kinetic energy, centre of mass x, y, z}. */
+int sph_conserved(sph_handle h, double* out);
+
+/* Runs the reference's debug validators (SimulationEngine.h:115-282) on the
+ * device. Returns SPH_OK, or SPH_E_VALIDATION with the SPH_V_* bits in *bits. */
+int sph_validate(sph_handle h, uint32_t* bits);
+
+/* SimulationEngine::GetSimulationTime (SimulationEngine.h:48). */
+float sph_simulation_time(sph_handle h);
+
+/* SimulationEngine::Clear (SimulationEngine.h:40; SimulationEngineCPU.cpp:365-395):
+ * waits for outstanding work and releases all device state. */
+int sph_clear(sph_handle h);
+
+/* Message of the last failure on this handle ("" if none). */
+const char* sph_last_error(sph_handle h);
+
+/* ParticleBufferManager::GetParticleCount (ParticleBufferManager.h:20) of the
+ * dynamic and static managers, summed over scenes. */
+uint64_t sph_particle_count(sph_handle h);
+uint64_t sph_static_particle_count(sph_handle h);
+uint32_t sph_scene_count(sph_handle h);
+/* Number of dynamic buckets M of one scene (the table has M + 1 entries). */
+uint64_t sph_hash_map_size(sph_handle h, uint32_t scene);
+
+/* Per-stage timing with hipEvents on the engine's stream (the reference's
+ * PerformanceProfile, SimulationEngineGPU.cpp:204-273). stage_mask has bit
+ * (1 << SPH_STAGE_*) set for every stage whose launches are bracketed by events
+ * (0 turns profiling off, SPH_PROFILE_ALL times every stage); sph_stage_times
+ * returns the summed milliseconds and launch count per stage since the last
+ * sph_set_profiling call. Steps are launched directly, not from hipGraphs,
+ * while profiling is on. */
+#define SPH_PROFILE_ALL 0xFu
+int sph_set_profiling(sph_handle h, uint32_t stage_mask);
+int sph_stage_times(sph_handle h, double* ms, uint64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPH_HIP_H */

@@ -1,0 +1,62 @@
+"""Build recipe: compiles the gfx950 engine library in-tree and the CPU oracle.
+
+    python -m sphfluidsimulation_amd.build
+
+produces sphfluidsimulation_amd/libsph_hip.so (hipcc, --offload-arch=gfx950)
+and oracle/libsph_oracle.so (g++; test infrastructure only).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIB = os.path.join(PKG, "libsph_hip.so")
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    # No packed FP32 instructions (v_pk_*_f32) anywhere in the device code.
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def _run(cmd, cwd=None):
+    print("+ " + " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if out.strip():
+        print(out, flush=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}")
+
+
+def build_hip():
+    src = os.path.join(CSRC, "sph_engine.hip")
+    tmp = LIB + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-o", tmp, src])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle():
+    _run(["make", "-B", "-s", "libsph_oracle.so"], cwd=ORACLE_DIR)
+    return os.path.join(ORACLE_DIR, "libsph_oracle.so")
+
+
+def build_all():
+    build_hip()
+    build_oracle()
+
+
+if __name__ == "__main__":
+    build_all()
+    sys.exit(0)

@@ -1,0 +1,883 @@
+// Host side of the MI355X SPH step engine and its C ABI (include/sph_hip.h).
+//
+// The engine mirrors SPH::SimulationEngine (SPHFluidSimulation/include/SPH/Core/
+// SimulationEngine.h:35-48): Initialize sorts the scene once on the host, in the
+// order a one-thread reference run produces, and uploads it; Update enqueues the
+// four step kernels of sph_kernels.hpp per step on the engine's own stream
+// (optionally replayed from captured hipGraphs); readers copy the state back in
+// original order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sph_hip.h"
+#include "sph_kernels.hpp"
+
+using namespace sph;
+
+namespace {
+
+struct StageEvent {
+    int stage;
+    hipEvent_t start, stop;
+};
+
+}  // namespace
+
+struct sph_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool initialized = false;
+    uint32_t flags = 0;
+
+    // Sizes.
+    uint32_t nScenes = 0;
+    uint64_t N = 0, Ns = 0, L = 0, Ls = 0, nTris = 0;
+    uint32_t numBlocks = 0, numTiles = 0;
+    bool hasStatic = false, hasTris = false;
+    std::vector<SceneDev> hScenes;
+    std::vector<BlockDesc> hBlocks;
+    std::vector<uint32_t> hSceneBlocks;  // nScenes + 1 block offsets
+    std::vector<uint32_t> hStaticIds;    // original global wall index per sorted wall slot
+    std::vector<float> hTris;
+
+    // Device buffers.
+    SceneDev* dScenes = nullptr;
+    BlockDesc* dBlocks = nullptr;
+    uint32_t* dSceneBlocks = nullptr;
+    float4 *pos = nullptr, *vel = nullptr, *posT = nullptr, *velT = nullptr;
+    float* pres = nullptr;
+    uint32_t* ids[2] = {nullptr, nullptr};
+    uint32_t* origOfSlot = nullptr;
+    uint32_t* tab[2] = {nullptr, nullptr};
+    unsigned long long* status[2] = {nullptr, nullptr};
+    uint32_t* ticket = nullptr;
+    float4* spos = nullptr;
+    uint32_t* stab = nullptr;
+    uint32_t* staticScene = nullptr;
+    float* tris = nullptr;
+    float* dens = nullptr;
+    float4* acc = nullptr;
+    double* consPartial = nullptr;
+    double* consOut = nullptr;
+    uint32_t* valBits = nullptr;
+    uint32_t* valIdx = nullptr;  // 4 * nScenes: firstIdx, firstVal, lastIdx, lastVal (dynamic), then static
+    int cur = 0;                 // ids[cur] / tab[cur] hold the current sorted state
+    bool debugValid = false;
+    double simTime = 0.0;
+
+    // hipGraph replay: one graph per starting parity, keyed by dt.
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    float graphDt[2] = {0.f, 0.f};
+
+    // Profiling.
+    uint32_t profiling = 0;  // stage mask
+    std::vector<StageEvent> pending;
+    std::vector<hipEvent_t> eventPool;
+    double stageMs[SPH_STAGE_COUNT] = {0, 0, 0, 0};
+    uint64_t stageLaunches[SPH_STAGE_COUNT] = {0, 0, 0, 0};
+};
+
+#define SPH_HIP(h, expr)                                                                      \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) {                                                               \
+            (h)->err = std::string(#expr) + ": " + hipGetErrorString(e_);                     \
+            return SPH_E_HIP;                                                                 \
+        }                                                                                     \
+    } while (0)
+
+namespace {
+
+template <class T>
+int dalloc(sph_engine* h, T** p, uint64_t count) {
+    *p = nullptr;
+    if (count == 0) return SPH_OK;
+    SPH_HIP(h, hipMalloc((void**)p, sizeof(T) * count));
+    return SPH_OK;
+}
+
+template <class T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+void release(sph_engine* h) {
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (auto& g : h->graph) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+    for (auto& pe : h->pending) {
+        h->eventPool.push_back(pe.start);
+        h->eventPool.push_back(pe.stop);
+    }
+    h->pending.clear();
+    dfree(h->dScenes);
+    dfree(h->dBlocks);
+    dfree(h->dSceneBlocks);
+    dfree(h->pos);
+    dfree(h->vel);
+    dfree(h->posT);
+    dfree(h->velT);
+    dfree(h->pres);
+    dfree(h->ids[0]);
+    dfree(h->ids[1]);
+    dfree(h->origOfSlot);
+    dfree(h->tab[0]);
+    dfree(h->tab[1]);
+    dfree(h->status[0]);
+    dfree(h->status[1]);
+    dfree(h->ticket);
+    dfree(h->spos);
+    dfree(h->stab);
+    dfree(h->staticScene);
+    dfree(h->tris);
+    dfree(h->dens);
+    dfree(h->acc);
+    dfree(h->consPartial);
+    dfree(h->consOut);
+    dfree(h->valBits);
+    dfree(h->valIdx);
+    h->hScenes.clear();
+    h->hBlocks.clear();
+    h->hSceneBlocks.clear();
+    h->hStaticIds.clear();
+    h->hTris.clear();
+    h->N = h->Ns = h->L = h->Ls = h->nTris = 0;
+    h->nScenes = 0;
+    h->numBlocks = h->numTiles = 0;
+    h->hasStatic = h->hasTris = false;
+    h->cur = 0;
+    h->simTime = 0.0;
+    h->debugValid = false;
+    h->initialized = false;
+}
+
+// Reference counting sort of one scene's keys, single-threaded, as
+// SimulationEngineCPU.cpp:55-131 with one thread (and SimulationEngine.h:79-100
+// for walls): histogram, inclusive scan, then a pre-decrement scatter in input
+// order, so equal keys end in descending input order and table[b] is the start
+// of bucket b. Returns slot[i] for input i.
+void counting_sort(const std::vector<uint32_t>& keys, uint64_t M, std::vector<uint32_t>& table,
+                   std::vector<uint32_t>& slotOf) {
+    const uint64_t n = keys.size();
+    table.assign(M + 1, 0u);
+    for (uint64_t i = 0; i < n; ++i) ++table[keys[i]];
+    uint32_t sum = 0;
+    for (uint64_t b = 0; b <= M; ++b) {
+        sum += table[b];
+        table[b] = sum;
+    }
+    slotOf.resize(n);
+    for (uint64_t i = 0; i < n; ++i) slotOf[i] = --table[keys[i]];
+}
+
+uint32_t key_of(const float* p, float h, uint64_t M) {
+    const uint32_t code = morton3(cell_coord(p[0], h), cell_coord(p[1], h), cell_coord(p[2], h));
+    return (uint32_t)(code % M);
+}
+
+hipEvent_t take_event(sph_engine* h) {
+    if (!h->eventPool.empty()) {
+        hipEvent_t e = h->eventPool.back();
+        h->eventPool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Folds finished stage events into the totals.
+int collect_events(sph_engine* h) {
+    for (auto& pe : h->pending) {
+        SPH_HIP(h, hipEventSynchronize(pe.stop));
+        float ms = 0.f;
+        SPH_HIP(h, hipEventElapsedTime(&ms, pe.start, pe.stop));
+        h->stageMs[pe.stage] += ms;
+        h->stageLaunches[pe.stage] += 1;
+        h->eventPool.push_back(pe.start);
+        h->eventPool.push_back(pe.stop);
+    }
+    h->pending.clear();
+    return SPH_OK;
+}
+
+// Enqueues one step starting from parity h->cur (does not flip it).
+int enqueue_step(sph_engine* h, float dt, bool capture) {
+    const int c = h->cur;
+    const bool dbg = (h->flags & SPH_FLAG_RECORD_DEBUG) != 0;
+    const uint32_t prof = capture ? 0u : h->profiling;
+    const dim3 grid(h->numBlocks), block(kBlock);
+    StageEvent ev{};
+    bool timing = false;
+    auto begin = [&](int stage) -> int {
+        timing = (prof >> stage) & 1u;
+        if (!timing) return SPH_OK;
+        ev.stage = stage;
+        ev.start = take_event(h);
+        ev.stop = take_event(h);
+        if (!ev.start || !ev.stop) {
+            h->err = "hipEventCreate failed";
+            return SPH_E_HIP;
+        }
+        SPH_HIP(h, hipEventRecord(ev.start, h->stream));
+        return SPH_OK;
+    };
+    auto end = [&]() -> int {
+        if (!timing) return SPH_OK;
+        SPH_HIP(h, hipEventRecord(ev.stop, h->stream));
+        h->pending.push_back(ev);
+        return SPH_OK;
+    };
+    int rc;
+    // Density / pressure.
+    if ((rc = begin(SPH_STAGE_DENSITY))) return rc;
+    if (h->hasStatic) {
+        if (dbg)
+            hipLaunchKernelGGL((k_density<true, true>), grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->pos,
+                               h->tab[c], h->spos, h->stab, h->pres, h->dens);
+        else
+            hipLaunchKernelGGL((k_density<true, false>), grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->pos,
+                               h->tab[c], h->spos, h->stab, h->pres, h->dens);
+    } else {
+        if (dbg)
+            hipLaunchKernelGGL((k_density<false, true>), grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->pos,
+                               h->tab[c], h->spos, h->stab, h->pres, h->dens);
+        else
+            hipLaunchKernelGGL((k_density<false, false>), grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->pos,
+                               h->tab[c], h->spos, h->stab, h->pres, h->dens);
+    }
+    SPH_HIP(h, hipGetLastError());
+    if ((rc = end())) return rc;
+
+    // Forces, integration, new keys, histogram into the other table.
+    if ((rc = begin(SPH_STAGE_FORCE))) return rc;
+#define SPH_FORCE(S, D, T)                                                                                        \
+    hipLaunchKernelGGL((k_force<S, D, T>), grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->pos, h->vel,     \
+                       h->pres, h->tab[c], h->spos, h->stab, h->ids[c], h->tris, dt, h->posT, h->velT, h->tab[c ^ 1], \
+                       h->acc)
+    if (h->hasStatic) {
+        if (dbg) {
+            if (h->hasTris) SPH_FORCE(true, true, true); else SPH_FORCE(true, true, false);
+        } else {
+            if (h->hasTris) SPH_FORCE(true, false, true); else SPH_FORCE(true, false, false);
+        }
+    } else {
+        if (dbg) {
+            if (h->hasTris) SPH_FORCE(false, true, true); else SPH_FORCE(false, true, false);
+        } else {
+            if (h->hasTris) SPH_FORCE(false, false, true); else SPH_FORCE(false, false, false);
+        }
+    }
+#undef SPH_FORCE
+    SPH_HIP(h, hipGetLastError());
+    if ((rc = end())) return rc;
+
+    // Scan the histogram; clear the table read this step for the next histogram.
+    if ((rc = begin(SPH_STAGE_SCAN))) return rc;
+    hipLaunchKernelGGL(k_scan, dim3(h->numTiles), dim3(kScanThreads), 0, h->stream, h->tab[c ^ 1], h->tab[c],
+                       (uint32_t)h->L, h->status[c], h->status[c ^ 1], h->ticket, h->numTiles);
+    SPH_HIP(h, hipGetLastError());
+    if ((rc = end())) return rc;
+
+    // Move the state into bucket order.
+    if ((rc = begin(SPH_STAGE_SCATTER))) return rc;
+    hipLaunchKernelGGL(k_scatter, grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->posT, h->velT, h->ids[c],
+                       h->tab[c ^ 1], h->pos, h->vel, h->ids[c ^ 1]);
+    SPH_HIP(h, hipGetLastError());
+    if ((rc = end())) return rc;
+    return SPH_OK;
+}
+
+int capture_graph(sph_engine* h, int parity, float dt) {
+    if (h->graph[parity]) {
+        SPH_HIP(h, hipGraphExecDestroy(h->graph[parity]));
+        h->graph[parity] = nullptr;
+    }
+    const int saved = h->cur;
+    h->cur = parity;
+    SPH_HIP(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_step(h, dt, true);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    h->cur = saved;
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) {
+        h->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+        return SPH_E_HIP;
+    }
+    e = hipGraphInstantiate(&h->graph[parity], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        h->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
+        h->graph[parity] = nullptr;
+        return SPH_E_HIP;
+    }
+    h->graphDt[parity] = dt;
+    return SPH_OK;
+}
+
+bool valid(sph_handle h) { return h != nullptr; }
+
+}  // namespace
+
+extern "C" {
+
+const char* sph_system_implementation_name(void) { return "HIP"; }
+
+int sph_create(sph_handle* out, int device) {
+    if (!out) return SPH_E_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return SPH_E_NO_DEVICE;
+    if (device < 0 || device >= count) return SPH_E_INVALID;
+    sph_engine* h = new sph_engine;
+    h->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return SPH_E_HIP;
+    }
+    *out = h;
+    return SPH_OK;
+}
+
+int sph_destroy(sph_handle h) {
+    if (!valid(h)) return SPH_E_INVALID;
+    (void)hipSetDevice(h->device);
+    release(h);
+    for (auto e : h->eventPool) (void)hipEventDestroy(e);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return SPH_OK;
+}
+
+int sph_clear(sph_handle h) {
+    if (!valid(h)) return SPH_E_INVALID;
+    SPH_HIP(h, hipSetDevice(h->device));
+    release(h);
+    return SPH_OK;
+}
+
+const char* sph_last_error(sph_handle h) { return h ? h->err.c_str() : "invalid handle"; }
+
+int sph_initialize(sph_handle h, const sph_params* params, const float* dyn_pos, const float* dyn_vel, uint64_t n,
+                   const float* static_pos, uint64_t n_s, uint32_t flags) {
+    const uint64_t dOff[2] = {0, n};
+    const uint64_t sOff[2] = {0, n_s};
+    return sph_batch_initialize(h, 1, params, dOff, dyn_pos, dyn_vel, sOff, static_pos, flags);
+}
+
+int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* params, const uint64_t* dyn_offsets,
+                         const float* dyn_pos, const float* dyn_vel, const uint64_t* static_offsets,
+                         const float* static_pos, uint32_t flags) {
+    if (!valid(h)) return SPH_E_INVALID;
+    if (n_scenes == 0 || !params || !dyn_offsets || !static_offsets) {
+        h->err = "sph_batch_initialize: no scenes or missing offsets/params";
+        return SPH_E_INVALID;
+    }
+    SPH_HIP(h, hipSetDevice(h->device));
+    release(h);  // Initialize starts with Clear (SimulationEngineCPU.cpp:398)
+    h->err.clear();
+    h->flags = flags;
+    const uint64_t N = dyn_offsets[n_scenes], Ns = static_offsets[n_scenes];
+    if ((N && !dyn_pos) || (Ns && !static_pos)) {
+        h->err = "sph_batch_initialize: missing particle arrays";
+        return SPH_E_INVALID;
+    }
+    for (uint32_t s = 0; s < n_scenes; ++s) {
+        if (dyn_offsets[s + 1] < dyn_offsets[s] || static_offsets[s + 1] < static_offsets[s] ||
+            !(params[s].maxInteractionDistance > 0.f) || params[s].gasConstant == 0.f) {
+            h->err = "sph_batch_initialize: bad offsets or parameters for a scene";
+            return SPH_E_INVALID;
+        }
+    }
+    uint64_t L = 0;
+    for (uint32_t s = 0; s < n_scenes; ++s) L += 2 * (dyn_offsets[s + 1] - dyn_offsets[s]) + 1;
+    if (N >= (1ull << 31) || L >= (1ull << 32) - kScanTile || Ns >= (1ull << 32)) {
+        h->err = "sph_batch_initialize: particle counts exceed the 32-bit index range";
+        return SPH_E_INVALID;
+    }
+    h->nScenes = n_scenes;
+    h->N = N;
+    h->Ns = Ns;
+    h->L = L;
+
+    std::vector<float4> hPos(N), hVel(N), hSpos(Ns);
+    std::vector<uint32_t> hIds(N), hOrig(N), hTab(L), hStab, hStaticScene(Ns);
+    h->hStaticIds.assign(Ns, 0);
+    h->hScenes.resize(n_scenes);
+    h->hSceneBlocks.assign(n_scenes + 1, 0);
+    h->hBlocks.clear();
+    uint64_t tBegin = 0, stBegin = 0;
+    std::vector<uint32_t> keys, table, slotOf;
+    for (uint32_t s = 0; s < n_scenes; ++s) {
+        const sph_params& P = params[s];
+        SceneDev& S = h->hScenes[s];
+        memset(&S, 0, sizeof(S));
+        S.h = P.maxInteractionDistance;
+        S.h2 = S.h * S.h;
+        S.mass = P.particleMass;
+        S.k = P.gasConstant;
+        S.rho0 = P.restDensity;
+        S.mu = P.viscosity;
+        S.gx = P.gravityX;
+        S.gy = P.gravityY;
+        S.gz = P.gravityZ;
+        S.C = kernel_constant(S.h);                     // SimulationEngineCPU.cpp:406
+        S.selfDensity = S.mass * d0_at_zero(S.h) * S.C;  // SimulationEngineCPU.cpp:407
+        const uint64_t p0 = dyn_offsets[s], pn = dyn_offsets[s + 1] - p0;
+        const uint64_t s0 = static_offsets[s], sn = static_offsets[s + 1] - s0;
+        S.pBegin = (uint32_t)p0;
+        S.pCount = (uint32_t)pn;
+        S.tBegin = (uint32_t)tBegin;
+        S.M = (uint32_t)(2 * pn);
+        S.magicM = pn ? fastmod_magic(S.M) : 0;
+        S.sBegin = (uint32_t)s0;
+        S.sCount = (uint32_t)sn;
+        S.stBegin = (uint32_t)stBegin;
+        S.Ms = (uint32_t)sn;
+        S.magicMs = sn ? fastmod_magic(S.Ms) : 0;
+        // Walls: SimulationEngine.h:79-114, M_s = N_s (SimulationEngineCPU.cpp:468).
+        if (sn) {
+            keys.resize(sn);
+            for (uint64_t i = 0; i < sn; ++i) keys[i] = key_of(static_pos + 3 * (s0 + i), S.h, S.Ms);
+            counting_sort(keys, S.Ms, table, slotOf);
+            for (uint64_t i = 0; i < sn; ++i) {
+                const float* q = static_pos + 3 * (s0 + i);
+                hSpos[s0 + slotOf[i]] = make_float4(q[0], q[1], q[2], 0.f);
+                h->hStaticIds[s0 + slotOf[i]] = (uint32_t)(s0 + i);
+                hStaticScene[s0 + slotOf[i]] = s;
+            }
+            for (uint64_t b = 0; b <= sn; ++b) hStab.push_back((uint32_t)(s0 + table[b]));
+            stBegin += sn + 1;
+        }
+        // Dynamic: M = 2N (SimulationEngineCPU.cpp:486-487), one-thread order.
+        if (pn) {
+            keys.resize(pn);
+            for (uint64_t i = 0; i < pn; ++i) keys[i] = key_of(dyn_pos + 3 * (p0 + i), S.h, S.M);
+            counting_sort(keys, S.M, table, slotOf);
+            for (uint64_t i = 0; i < pn; ++i) {
+                const uint64_t g = p0 + slotOf[i];
+                const float* q = dyn_pos + 3 * (p0 + i);
+                hPos[g] = make_float4(q[0], q[1], q[2], 0.f);
+                const float* w = dyn_vel ? dyn_vel + 3 * (p0 + i) : nullptr;
+                float keyBits;
+                memcpy(&keyBits, &keys[i], sizeof(float));
+                hVel[g] = make_float4(w ? w[0] : 0.f, w ? w[1] : 0.f, w ? w[2] : 0.f, keyBits);
+                hIds[g] = (uint32_t)g;
+                hOrig[g] = (uint32_t)(p0 + i);
+            }
+            for (uint64_t b = 0; b <= S.M; ++b) hTab[tBegin + b] = (uint32_t)(p0 + table[b]);
+        } else {
+            hTab[tBegin] = (uint32_t)p0;
+        }
+        tBegin += 2 * pn + 1;
+        // Workgroups of this scene.
+        h->hSceneBlocks[s] = (uint32_t)h->hBlocks.size();
+        for (uint64_t b = 0; b < pn; b += kBlock) {
+            BlockDesc bd;
+            bd.scene = s;
+            bd.begin = (uint32_t)(p0 + b);
+            bd.end = (uint32_t)(p0 + std::min<uint64_t>(pn, b + kBlock));
+            bd.pad = 0;
+            h->hBlocks.push_back(bd);
+        }
+    }
+    h->hSceneBlocks[n_scenes] = (uint32_t)h->hBlocks.size();
+    h->Ls = hStab.size();
+    h->numBlocks = (uint32_t)h->hBlocks.size();
+    h->numTiles = (uint32_t)((L + kScanTile - 1) / kScanTile);
+    h->hasStatic = Ns > 0;
+
+    int rc;
+#define SPH_ALLOC(p, n)                       \
+    if ((rc = dalloc(h, &(p), (n)))) return rc;
+    SPH_ALLOC(h->dScenes, n_scenes);
+    SPH_ALLOC(h->dBlocks, std::max<uint64_t>(h->numBlocks, 1));
+    SPH_ALLOC(h->dSceneBlocks, n_scenes + 1);
+    SPH_ALLOC(h->pos, std::max<uint64_t>(N, 1));
+    SPH_ALLOC(h->vel, std::max<uint64_t>(N, 1));
+    SPH_ALLOC(h->posT, std::max<uint64_t>(N, 1));
+    SPH_ALLOC(h->velT, std::max<uint64_t>(N, 1));
+    SPH_ALLOC(h->pres, std::max<uint64_t>(N, 1));
+    SPH_ALLOC(h->ids[0], std::max<uint64_t>(N, 1));
+    SPH_ALLOC(h->ids[1], std::max<uint64_t>(N, 1));
+    SPH_ALLOC(h->origOfSlot, std::max<uint64_t>(N, 1));
+    SPH_ALLOC(h->tab[0], L);
+    SPH_ALLOC(h->tab[1], L);
+    SPH_ALLOC(h->status[0], h->numTiles);
+    SPH_ALLOC(h->status[1], h->numTiles);
+    SPH_ALLOC(h->ticket, 1);
+    SPH_ALLOC(h->spos, std::max<uint64_t>(Ns, 1));
+    SPH_ALLOC(h->stab, std::max<uint64_t>(h->Ls, 1));
+    SPH_ALLOC(h->staticScene, std::max<uint64_t>(Ns, 1));
+    SPH_ALLOC(h->consPartial, 8ull * std::max<uint32_t>(h->numBlocks, 1));
+    SPH_ALLOC(h->consOut, 8ull * n_scenes);
+    SPH_ALLOC(h->valBits, 1);
+    SPH_ALLOC(h->valIdx, 8ull * n_scenes);
+    if (flags & SPH_FLAG_RECORD_DEBUG) {
+        SPH_ALLOC(h->dens, std::max<uint64_t>(N, 1));
+        SPH_ALLOC(h->acc, std::max<uint64_t>(N, 1));
+    }
+#undef SPH_ALLOC
+    SPH_HIP(h, hipMemcpy(h->dScenes, h->hScenes.data(), sizeof(SceneDev) * n_scenes, hipMemcpyHostToDevice));
+    if (h->numBlocks)
+        SPH_HIP(h, hipMemcpy(h->dBlocks, h->hBlocks.data(), sizeof(BlockDesc) * h->numBlocks, hipMemcpyHostToDevice));
+    SPH_HIP(h, hipMemcpy(h->dSceneBlocks, h->hSceneBlocks.data(), sizeof(uint32_t) * (n_scenes + 1),
+                         hipMemcpyHostToDevice));
+    if (N) {
+        SPH_HIP(h, hipMemcpy(h->pos, hPos.data(), sizeof(float4) * N, hipMemcpyHostToDevice));
+        SPH_HIP(h, hipMemcpy(h->vel, hVel.data(), sizeof(float4) * N, hipMemcpyHostToDevice));
+        SPH_HIP(h, hipMemcpy(h->ids[0], hIds.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice));
+        SPH_HIP(h, hipMemcpy(h->origOfSlot, hOrig.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice));
+        SPH_HIP(h, hipMemset(h->pres, 0, sizeof(float) * N));
+    }
+    SPH_HIP(h, hipMemcpy(h->tab[0], hTab.data(), sizeof(uint32_t) * L, hipMemcpyHostToDevice));
+    SPH_HIP(h, hipMemset(h->tab[1], 0, sizeof(uint32_t) * L));
+    SPH_HIP(h, hipMemset(h->status[0], 0, sizeof(unsigned long long) * h->numTiles));
+    SPH_HIP(h, hipMemset(h->status[1], 0, sizeof(unsigned long long) * h->numTiles));
+    SPH_HIP(h, hipMemset(h->ticket, 0, sizeof(uint32_t)));
+    if (Ns) {
+        SPH_HIP(h, hipMemcpy(h->spos, hSpos.data(), sizeof(float4) * Ns, hipMemcpyHostToDevice));
+        SPH_HIP(h, hipMemcpy(h->stab, hStab.data(), sizeof(uint32_t) * h->Ls, hipMemcpyHostToDevice));
+        SPH_HIP(h, hipMemcpy(h->staticScene, hStaticScene.data(), sizeof(uint32_t) * Ns, hipMemcpyHostToDevice));
+    }
+    // Validation index lists: dynamic first/last entries per scene, then static.
+    std::vector<uint32_t> vi(8ull * n_scenes);
+    for (uint32_t s = 0; s < n_scenes; ++s) {
+        const SceneDev& S = h->hScenes[s];
+        vi[0 * n_scenes + s] = S.tBegin;
+        vi[1 * n_scenes + s] = S.pBegin;
+        vi[2 * n_scenes + s] = S.tBegin + S.M;
+        vi[3 * n_scenes + s] = S.pBegin + S.pCount;
+        vi[4 * n_scenes + s] = S.sCount ? S.stBegin : 0;
+        vi[5 * n_scenes + s] = S.sCount ? S.sBegin : (h->Ls ? hStab[0] : 0);
+        vi[6 * n_scenes + s] = S.sCount ? S.stBegin + S.Ms : 0;
+        vi[7 * n_scenes + s] = S.sCount ? S.sBegin + S.sCount : (h->Ls ? hStab[0] : 0);
+    }
+    SPH_HIP(h, hipMemcpy(h->valIdx, vi.data(), sizeof(uint32_t) * vi.size(), hipMemcpyHostToDevice));
+    SPH_HIP(h, hipDeviceSynchronize());
+    h->cur = 0;
+    h->simTime = 0.0;
+    h->initialized = true;
+    return SPH_OK;
+}
+
+int sph_set_triangles(sph_handle h, uint32_t scene, const float* tri9, uint64_t count) {
+    if (!valid(h)) return SPH_E_INVALID;
+    if (!h->initialized || scene >= h->nScenes) {
+        h->err = "sph_set_triangles: engine not initialized or scene out of range";
+        return SPH_E_STATE;
+    }
+    if (count && !tri9) return SPH_E_INVALID;
+    SPH_HIP(h, hipSetDevice(h->device));
+    SPH_HIP(h, hipStreamSynchronize(h->stream));
+    // Rebuild the concatenated triangle list with this scene's triangles replaced.
+    std::vector<float> all;
+    std::vector<SceneDev>& S = h->hScenes;
+    for (uint32_t s = 0; s < h->nScenes; ++s) {
+        const uint32_t oldBegin = S[s].triBegin, oldCount = S[s].triCount;
+        const uint32_t newBegin = (uint32_t)(all.size() / 9);
+        if (s == scene) {
+            all.insert(all.end(), tri9, tri9 + 9 * count);
+            S[s].triCount = (uint32_t)count;
+        } else {
+            all.insert(all.end(), h->hTris.begin() + 9ull * oldBegin, h->hTris.begin() + 9ull * (oldBegin + oldCount));
+        }
+        S[s].triBegin = newBegin;
+    }
+    h->hTris.swap(all);
+    h->nTris = h->hTris.size() / 9;
+    h->hasTris = h->nTris > 0;
+    dfree(h->tris);
+    int rc;
+    if ((rc = dalloc(h, &h->tris, std::max<uint64_t>(h->hTris.size(), 1)))) return rc;
+    if (h->nTris)
+        SPH_HIP(h, hipMemcpy(h->tris, h->hTris.data(), sizeof(float) * h->hTris.size(), hipMemcpyHostToDevice));
+    SPH_HIP(h, hipMemcpy(h->dScenes, S.data(), sizeof(SceneDev) * h->nScenes, hipMemcpyHostToDevice));
+    for (auto& g : h->graph) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+    return SPH_OK;
+}
+
+int sph_update(sph_handle h, float dt, uint32_t steps) {
+    if (!valid(h)) return SPH_E_INVALID;
+    if (!h->initialized) {
+        h->err = "Updating an uninitialized SPH system";  // SimulationEngineCPU.cpp:422-426
+        return SPH_E_STATE;
+    }
+    if (h->N == 0 || steps == 0) return SPH_OK;  // SimulationEngineCPU.cpp:428-429
+    SPH_HIP(h, hipSetDevice(h->device));
+    const bool useGraph = (h->flags & SPH_FLAG_HIP_GRAPH) && !h->profiling;
+    for (uint32_t s = 0; s < steps; ++s) {
+        if (useGraph) {
+            const int c = h->cur;
+            if (!h->graph[c] || h->graphDt[c] != dt) {
+                int rc = capture_graph(h, c, dt);
+                if (rc) return rc;
+            }
+            SPH_HIP(h, hipGraphLaunch(h->graph[c], h->stream));
+        } else {
+            int rc = enqueue_step(h, dt, false);
+            if (rc) return rc;
+        }
+        h->cur ^= 1;
+    }
+    h->debugValid = (h->flags & SPH_FLAG_RECORD_DEBUG) != 0;
+    h->simTime += (double)dt * steps;  // SimulationEngineCPU.cpp:453
+    return SPH_OK;
+}
+
+int sph_synchronize(sph_handle h) {
+    if (!valid(h)) return SPH_E_INVALID;
+    SPH_HIP(h, hipSetDevice(h->device));
+    SPH_HIP(h, hipStreamSynchronize(h->stream));
+    return SPH_OK;
+}
+
+int sph_read(sph_handle h, float* pos, float* vel, float* pressure) {
+    if (!valid(h)) return SPH_E_INVALID;
+    if (!h->initialized) {
+        h->err = "sph_read: engine not initialized";
+        return SPH_E_STATE;
+    }
+    if (h->N == 0) return SPH_OK;
+    SPH_HIP(h, hipSetDevice(h->device));
+    const uint64_t N = h->N;
+    float *dP = nullptr, *dV = nullptr, *dPr = nullptr;
+    uint32_t *dK = nullptr, *dI = nullptr;
+    int rc = SPH_OK;
+    if ((rc = dalloc(h, &dP, 3 * N)) || (rc = dalloc(h, &dV, 3 * N)) || (rc = dalloc(h, &dPr, N)) ||
+        (rc = dalloc(h, &dK, N)) || (rc = dalloc(h, &dI, N))) {
+        dfree(dP), dfree(dV), dfree(dPr), dfree(dK), dfree(dI);
+        return rc;
+    }
+    hipLaunchKernelGGL(k_unsort, dim3((uint32_t)((N + 255) / 256)), dim3(256), 0, h->stream, h->pos, h->vel,
+                       h->ids[h->cur], h->origOfSlot, (uint32_t)N, dP, dV, dPr, dK, dI);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess && pos) e = hipMemcpy(pos, dP, sizeof(float) * 3 * N, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && vel) e = hipMemcpy(vel, dV, sizeof(float) * 3 * N, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && pressure) e = hipMemcpy(pressure, dPr, sizeof(float) * N, hipMemcpyDeviceToHost);
+    dfree(dP), dfree(dV), dfree(dPr), dfree(dK), dfree(dI);
+    if (e != hipSuccess) {
+        h->err = std::string("sph_read: ") + hipGetErrorString(e);
+        return SPH_E_HIP;
+    }
+    return SPH_OK;
+}
+
+namespace {
+// keys (original order) and/or bucket ids (sorted order) of the current state.
+int read_keys_ids(sph_engine* h, uint32_t* keys, uint32_t* bucketIds) {
+    if (!h->initialized) {
+        h->err = "engine not initialized";
+        return SPH_E_STATE;
+    }
+    if (h->N == 0) return SPH_OK;
+    SPH_HIP(h, hipSetDevice(h->device));
+    const uint64_t N = h->N;
+    float *dP = nullptr, *dV = nullptr, *dPr = nullptr;
+    uint32_t *dK = nullptr, *dI = nullptr;
+    int rc = SPH_OK;
+    if ((rc = dalloc(h, &dP, 3 * N)) || (rc = dalloc(h, &dV, 3 * N)) || (rc = dalloc(h, &dPr, N)) ||
+        (rc = dalloc(h, &dK, N)) || (rc = dalloc(h, &dI, N))) {
+        dfree(dP), dfree(dV), dfree(dPr), dfree(dK), dfree(dI);
+        return rc;
+    }
+    hipLaunchKernelGGL(k_unsort, dim3((uint32_t)((N + 255) / 256)), dim3(256), 0, h->stream, h->pos, h->vel,
+                       h->ids[h->cur], h->origOfSlot, (uint32_t)N, dP, dV, dPr, dK, dI);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess && keys) e = hipMemcpy(keys, dK, sizeof(uint32_t) * N, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && bucketIds) e = hipMemcpy(bucketIds, dI, sizeof(uint32_t) * N, hipMemcpyDeviceToHost);
+    dfree(dP), dfree(dV), dfree(dPr), dfree(dK), dfree(dI);
+    if (e != hipSuccess) {
+        h->err = std::string("read: ") + hipGetErrorString(e);
+        return SPH_E_HIP;
+    }
+    return SPH_OK;
+}
+}  // namespace
+
+int sph_read_keys(sph_handle h, uint32_t* keys) {
+    if (!valid(h) || !keys) return SPH_E_INVALID;
+    return read_keys_ids(h, keys, nullptr);
+}
+
+int sph_read_bucket_ids(sph_handle h, uint32_t* ids) {
+    if (!valid(h) || !ids) return SPH_E_INVALID;
+    return read_keys_ids(h, nullptr, ids);
+}
+
+int sph_read_cell_starts(sph_handle h, uint32_t* starts) {
+    if (!valid(h) || !starts) return SPH_E_INVALID;
+    if (!h->initialized) {
+        h->err = "sph_read_cell_starts: engine not initialized";
+        return SPH_E_STATE;
+    }
+    SPH_HIP(h, hipSetDevice(h->device));
+    SPH_HIP(h, hipStreamSynchronize(h->stream));
+    SPH_HIP(h, hipMemcpy(starts, h->tab[h->cur], sizeof(uint32_t) * h->L, hipMemcpyDeviceToHost));
+    for (uint32_t s = 0; s < h->nScenes; ++s) {
+        const SceneDev& S = h->hScenes[s];
+        for (uint64_t b = 0; b <= S.M; ++b) starts[S.tBegin + b] -= S.pBegin;
+    }
+    return SPH_OK;
+}
+
+int sph_read_static(sph_handle h, uint32_t* starts, uint32_t* ids) {
+    if (!valid(h)) return SPH_E_INVALID;
+    if (!h->initialized) {
+        h->err = "sph_read_static: engine not initialized";
+        return SPH_E_STATE;
+    }
+    SPH_HIP(h, hipSetDevice(h->device));
+    if (starts && h->Ls) {
+        SPH_HIP(h, hipMemcpy(starts, h->stab, sizeof(uint32_t) * h->Ls, hipMemcpyDeviceToHost));
+        for (uint32_t s = 0; s < h->nScenes; ++s) {
+            const SceneDev& S = h->hScenes[s];
+            if (!S.sCount) continue;
+            for (uint64_t b = 0; b <= S.Ms; ++b) starts[S.stBegin + b] -= S.sBegin;
+        }
+    }
+    if (ids && h->Ns) memcpy(ids, h->hStaticIds.data(), sizeof(uint32_t) * h->Ns);
+    return SPH_OK;
+}
+
+int sph_read_debug(sph_handle h, float* density, float* accel) {
+    if (!valid(h)) return SPH_E_INVALID;
+    if (!(h->flags & SPH_FLAG_RECORD_DEBUG) || !h->debugValid) {
+        h->err = "sph_read_debug: needs SPH_FLAG_RECORD_DEBUG and at least one step";
+        return SPH_E_STATE;
+    }
+    SPH_HIP(h, hipSetDevice(h->device));
+    const uint64_t N = h->N;
+    float *dD = nullptr, *dA = nullptr;
+    int rc;
+    if ((rc = dalloc(h, &dD, N)) || (rc = dalloc(h, &dA, 3 * N))) {
+        dfree(dD), dfree(dA);
+        return rc;
+    }
+    // The debug arrays are in the order of the last step's input: ids[cur ^ 1].
+    hipLaunchKernelGGL(k_unsort_debug, dim3((uint32_t)((N + 255) / 256)), dim3(256), 0, h->stream, h->dens, h->acc,
+                       h->ids[h->cur ^ 1], h->origOfSlot, (uint32_t)N, dD, dA);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess && density) e = hipMemcpy(density, dD, sizeof(float) * N, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && accel) e = hipMemcpy(accel, dA, sizeof(float) * 3 * N, hipMemcpyDeviceToHost);
+    dfree(dD), dfree(dA);
+    if (e != hipSuccess) {
+        h->err = std::string("sph_read_debug: ") + hipGetErrorString(e);
+        return SPH_E_HIP;
+    }
+    return SPH_OK;
+}
+
+int sph_conserved(sph_handle h, double* out) {
+    if (!valid(h) || !out) return SPH_E_INVALID;
+    if (!h->initialized) {
+        h->err = "sph_conserved: engine not initialized";
+        return SPH_E_STATE;
+    }
+    SPH_HIP(h, hipSetDevice(h->device));
+    if (h->numBlocks)
+        hipLaunchKernelGGL(k_conserved_partial, dim3(h->numBlocks), dim3(kBlock), 0, h->stream, h->dBlocks, h->pos,
+                           h->vel, h->consPartial);
+    hipLaunchKernelGGL(k_conserved_final, dim3(h->nScenes), dim3(64), 0, h->stream, h->dSceneBlocks, h->dScenes,
+                       h->consPartial, h->consOut);
+    SPH_HIP(h, hipGetLastError());
+    SPH_HIP(h, hipStreamSynchronize(h->stream));
+    SPH_HIP(h, hipMemcpy(out, h->consOut, sizeof(double) * 8 * h->nScenes, hipMemcpyDeviceToHost));
+    return SPH_OK;
+}
+
+int sph_validate(sph_handle h, uint32_t* bits) {
+    if (!valid(h)) return SPH_E_INVALID;
+    if (!h->initialized) {
+        h->err = "sph_validate: engine not initialized";
+        return SPH_E_STATE;
+    }
+    SPH_HIP(h, hipSetDevice(h->device));
+    SPH_HIP(h, hipMemsetAsync(h->valBits, 0, sizeof(uint32_t), h->stream));
+    const uint32_t ns = h->nScenes;
+    if (h->numBlocks)
+        hipLaunchKernelGGL(k_validate_particles, dim3(h->numBlocks), dim3(kBlock), 0, h->stream, h->dBlocks,
+                           h->dScenes, h->pos, h->vel, h->tab[h->cur], h->valBits);
+    {
+        const uint64_t n = std::max<uint64_t>(h->L, ns);
+        hipLaunchKernelGGL(k_validate_table, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, h->stream,
+                           h->tab[h->cur], (uint32_t)h->L, h->valIdx, h->valIdx + ns, h->valIdx + 2 * ns,
+                           h->valIdx + 3 * ns, ns, 0u, h->valBits);
+    }
+    if (h->Ns) {
+        // Static table: same checks, reported under SPH_V_STATIC.
+        const uint64_t n = std::max<uint64_t>(h->Ls, ns);
+        hipLaunchKernelGGL(k_validate_table, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, h->stream, h->stab,
+                           (uint32_t)h->Ls, h->valIdx + 4 * ns, h->valIdx + 5 * ns, h->valIdx + 6 * ns,
+                           h->valIdx + 7 * ns, ns, 4u, h->valBits);
+        hipLaunchKernelGGL(k_validate_static, dim3((uint32_t)((h->Ns + 255) / 256)), dim3(256), 0, h->stream,
+                           h->dScenes, ns, h->staticScene, h->spos, h->stab, (uint32_t)h->Ns, h->valBits);
+    }
+    SPH_HIP(h, hipGetLastError());
+    uint32_t b = 0;
+    SPH_HIP(h, hipMemcpyAsync(&b, h->valBits, sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
+    SPH_HIP(h, hipStreamSynchronize(h->stream));
+    // Static table bits were shifted by 4 into 0x40..0x200; fold them into SPH_V_STATIC.
+    uint32_t out = b & 0x3Fu;
+    if (b & ~0x3Fu) out |= SPH_V_STATIC;
+    if (bits) *bits = out;
+    return out ? SPH_E_VALIDATION : SPH_OK;
+}
+
+float sph_simulation_time(sph_handle h) { return h ? (float)h->simTime : 0.f; }
+
+uint64_t sph_particle_count(sph_handle h) { return h ? h->N : 0; }
+uint64_t sph_static_particle_count(sph_handle h) { return h ? h->Ns : 0; }
+uint32_t sph_scene_count(sph_handle h) { return h ? h->nScenes : 0; }
+uint64_t sph_hash_map_size(sph_handle h, uint32_t scene) {
+    if (!h || scene >= h->nScenes) return 0;
+    return h->hScenes[scene].M;
+}
+
+int sph_set_profiling(sph_handle h, uint32_t stage_mask) {
+    if (!valid(h)) return SPH_E_INVALID;
+    SPH_HIP(h, hipSetDevice(h->device));
+    SPH_HIP(h, hipStreamSynchronize(h->stream));
+    int rc = collect_events(h);
+    if (rc) return rc;
+    h->profiling = stage_mask & SPH_PROFILE_ALL;
+    for (int s = 0; s < SPH_STAGE_COUNT; ++s) {
+        h->stageMs[s] = 0;
+        h->stageLaunches[s] = 0;
+    }
+    return SPH_OK;
+}
+
+int sph_stage_times(sph_handle h, double* ms, uint64_t* launches) {
+    if (!valid(h)) return SPH_E_INVALID;
+    SPH_HIP(h, hipSetDevice(h->device));
+    int rc = collect_events(h);
+    if (rc) return rc;
+    for (int s = 0; s < SPH_STAGE_COUNT; ++s) {
+        if (ms) ms[s] = h->stageMs[s];
+        if (launches) launches[s] = h->stageLaunches[s];
+    }
+    return SPH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,203 @@
+"""SimulationEngineHIP: the Python host over the C ABI (include/sph_hip.h).
+
+It mirrors SPH::SimulationEngine (SPHFluidSimulation/include/SPH/Core/SimulationEngine.h:35-48):
+Clear, Initialize, Update, SystemImplementationName and GetSimulationTime keep the
+reference's names and meaning; the read* methods stand in for
+ParticleBufferManager::LockRead on the current buffer and return numpy arrays in
+original particle order. Errors raise SPHError (the reference logs and returns;
+the C ABI returns the code and this layer raises it).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .scenes import Scene, concat
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _ptr(a, ctype):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def _params(p) -> _lib.sph_params:
+    out = _lib.sph_params()
+    for name, _ in _lib.sph_params._fields_:
+        setattr(out, name, float(getattr(p, name)))
+    return out
+
+
+class SimulationEngineHIP:
+    """One engine per GPU stream; not thread-safe (like the reference's engines)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = _lib.load()
+        h = ctypes.c_void_p()
+        rc = self._lib.sph_create(ctypes.byref(h), int(device))
+        if rc != _lib.SPH_OK:
+            raise _lib.SPHError(rc, "sph_create failed (no HIP device?)")
+        self._h = h
+        self._scene_sizes = []  # (n, n_static, M) per scene
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.sph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        return _lib.check(self._h, rc)
+
+    # -- SimulationEngine interface ----------------------------------------
+    @staticmethod
+    def SystemImplementationName() -> str:
+        return _lib.load().sph_system_implementation_name().decode()
+
+    def Clear(self):
+        self._check(self._lib.sph_clear(self._h))
+        self._scene_sizes = []
+
+    def Initialize(self, scene, flags: int = 0):
+        """Initialize from one Scene or a list of Scenes (batched)."""
+        scenes = [scene] if isinstance(scene, Scene) else list(scene)
+        params, dyn_off, pos, vel, st_off, spos = concat(scenes)
+        arr = (_lib.sph_params * len(params))(*[_params(p) for p in params])
+        pos, vel, spos = _f32(pos), _f32(vel), _f32(spos)
+        self._check(self._lib.sph_batch_initialize(
+            self._h, len(scenes), arr, _ptr(dyn_off, ctypes.c_uint64), _ptr(pos, ctypes.c_float),
+            _ptr(vel, ctypes.c_float), _ptr(st_off, ctypes.c_uint64),
+            _ptr(spos, ctypes.c_float) if spos.size else None, int(flags)))
+        self._scene_sizes = [(sc.n, sc.n_static, int(self._lib.sph_hash_map_size(self._h, s)))
+                             for s, sc in enumerate(scenes)]
+        self._dyn_off = dyn_off.astype(np.int64)
+        self._st_off = st_off.astype(np.int64)
+
+    def Update(self, dt: float, steps: int = 1):
+        self._check(self._lib.sph_update(self._h, float(dt), int(steps)))
+
+    def GetSimulationTime(self) -> float:
+        return float(self._lib.sph_simulation_time(self._h))
+
+    # -- extras -----------------------------------------------------------
+    def set_triangles(self, scene: int, tris):
+        t = _f32(tris).reshape(-1, 9)
+        self._check(self._lib.sph_set_triangles(self._h, int(scene), _ptr(t, ctypes.c_float), t.shape[0]))
+
+    def synchronize(self):
+        self._check(self._lib.sph_synchronize(self._h))
+
+    @property
+    def n(self) -> int:
+        return int(self._lib.sph_particle_count(self._h))
+
+    @property
+    def n_static(self) -> int:
+        return int(self._lib.sph_static_particle_count(self._h))
+
+    @property
+    def n_scenes(self) -> int:
+        return int(self._lib.sph_scene_count(self._h))
+
+    def hash_map_size(self, scene: int = 0) -> int:
+        return int(self._lib.sph_hash_map_size(self._h, int(scene)))
+
+    def read(self):
+        n = self.n
+        pos = np.empty((n, 3), np.float32)
+        vel = np.empty((n, 3), np.float32)
+        p = np.empty(n, np.float32)
+        self._check(self._lib.sph_read(self._h, _ptr(pos, ctypes.c_float), _ptr(vel, ctypes.c_float),
+                                       _ptr(p, ctypes.c_float)))
+        return pos, vel, p
+
+    def read_keys(self):
+        k = np.empty(self.n, np.uint32)
+        self._check(self._lib.sph_read_keys(self._h, _ptr(k, ctypes.c_uint32)))
+        return k
+
+    def read_cell_starts(self):
+        """List of per-scene tables (M + 1 scene-local entries each)."""
+        total = sum(m + 1 for _, _, m in self._scene_sizes)
+        t = np.empty(total, np.uint32)
+        self._check(self._lib.sph_read_cell_starts(self._h, _ptr(t, ctypes.c_uint32)))
+        out, o = [], 0
+        for _, _, m in self._scene_sizes:
+            out.append(t[o:o + m + 1])
+            o += m + 1
+        return out
+
+    def read_bucket_ids(self):
+        ids = np.empty(self.n, np.uint32)
+        self._check(self._lib.sph_read_bucket_ids(self._h, _ptr(ids, ctypes.c_uint32)))
+        return ids
+
+    def read_static(self):
+        """(list of per-scene static tables, original wall index per sorted wall)."""
+        total = sum((ns + 1) if ns else 0 for _, ns, _ in self._scene_sizes)
+        t = np.empty(max(total, 1), np.uint32)
+        ids = np.empty(max(self.n_static, 1), np.uint32)
+        self._check(self._lib.sph_read_static(self._h, _ptr(t, ctypes.c_uint32), _ptr(ids, ctypes.c_uint32)))
+        out, o = [], 0
+        for _, ns, _ in self._scene_sizes:
+            if ns:
+                out.append(t[o:o + ns + 1])
+                o += ns + 1
+            else:
+                out.append(np.zeros(0, np.uint32))
+        return out, ids[:self.n_static]
+
+    def read_debug(self):
+        n = self.n
+        d = np.empty(n, np.float32)
+        a = np.empty((n, 3), np.float32)
+        self._check(self._lib.sph_read_debug(self._h, _ptr(d, ctypes.c_float), _ptr(a, ctypes.c_float)))
+        return d, a
+
+    def conserved(self):
+        out = np.empty((self.n_scenes, 8), np.float64)
+        self._check(self._lib.sph_conserved(self._h, _ptr(out, ctypes.c_double)))
+        return out
+
+    def validate(self) -> int:
+        bits = ctypes.c_uint32(0)
+        rc = self._lib.sph_validate(self._h, ctypes.byref(bits))
+        if rc not in (_lib.SPH_OK, _lib.SPH_E_VALIDATION):
+            self._check(rc)
+        return int(bits.value)
+
+    def set_profiling(self, stages=None):
+        """stages: iterable of stage names to time with hipEvents, True for all, None/False for none."""
+        if stages is True:
+            mask = _lib.SPH_PROFILE_ALL
+        elif not stages:
+            mask = 0
+        else:
+            mask = 0
+            for s in stages:
+                mask |= 1 << _lib.STAGES.index(s)
+        self._check(self._lib.sph_set_profiling(self._h, mask))
+
+    def stage_times(self):
+        ms = np.zeros(4, np.float64)
+        n = np.zeros(4, np.uint64)
+        self._check(self._lib.sph_stage_times(self._h, _ptr(ms, ctypes.c_double), _ptr(n, ctypes.c_uint64)))
+        return {name: (float(ms[i]), int(n[i])) for i, name in enumerate(_lib.STAGES)}
