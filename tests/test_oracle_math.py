@@ -1,0 +1,139 @@
+"""Known-answer checks of the oracle's scalar functions (SPHFunctions.cpp:65-223).
+
+Each check comes from mathematics, not from retyping the oracle's code: a
+bit-by-bit interleave for Morton codes, numpy's IEEE float32 division for cells,
+integrals and finite differences for the smoothing kernels, geometry for the
+triangle force.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as orc
+
+
+def interleave(x, y, z):
+    """Morton code by moving one bit at a time (bit k of x -> 3k, y -> 3k+1, z -> 3k+2)."""
+    out = 0
+    for k in range(10):
+        out |= ((x >> k) & 1) << (3 * k)
+        out |= ((y >> k) & 1) << (3 * k + 1)
+        out |= ((z >> k) & 1) << (3 * k + 2)
+    return out
+
+
+def test_morton_single_axis_all_values():
+    for v in range(1024):
+        want = interleave(v, 0, 0)
+        assert orc.morton(v, 0, 0) == want
+        assert orc.morton(0, v, 0) == want << 1
+        assert orc.morton(0, 0, v) == want << 2
+
+
+def test_morton_random_triples_and_aliasing():
+    rng = np.random.default_rng(0)
+    for x, y, z in rng.integers(0, 1024, size=(2000, 3)):
+        x, y, z = int(x), int(y), int(z)
+        code = orc.morton(x, y, z)
+        assert code == interleave(x, y, z)
+        assert code < (1 << 30)
+        # only the low 10 bits of a cell coordinate take part (SPHFunctions.cpp:93-95)
+        assert orc.morton(x + 1024, y + 5 * 1024, z + 1023 * 1024) == code
+    # a negative cell wraps through uint32: -1 -> 0xFFFFFFFF -> 1023 after masking
+    assert orc.morton(0xFFFFFFFF, 0, 0) == interleave(1023, 0, 0)
+    assert orc.morton(0xFFFFFFFE, 0xFFFFFFFF, 3) == interleave(1022, 1023, 3)
+
+
+def test_cells_are_ieee_floor_division_with_wrap():
+    rng = np.random.default_rng(1)
+    h = np.float32(0.1)
+    pos = rng.uniform(-5, 5, size=(20000, 3)).astype(np.float32)
+    # exact multiples of h and values just around them
+    k = np.arange(-50, 50, dtype=np.float32)[:, None] * h
+    pos = np.concatenate([pos, np.repeat(k, 3, axis=1), np.nextafter(np.repeat(k, 3, axis=1), np.float32(-1e9))])
+    got = orc.cells(pos, float(h))
+    q = (pos / h).astype(np.float32)  # IEEE float32 division
+    want = np.floor(q).astype(np.int64).astype(np.int32).view(np.uint32)
+    np.testing.assert_array_equal(got, want)
+    assert orc.cells(np.float32([[-0.05, -0.1, -1e-9]]), 0.1).tolist() == [[0xFFFFFFFF, 0xFFFFFFFF, 0xFFFFFFFF]]
+
+
+def test_keys_are_morton_mod_m():
+    rng = np.random.default_rng(2)
+    pos = rng.uniform(-1, 3, size=(3000, 3)).astype(np.float32)
+    for M in (1, 7, 32000, 2 ** 21):
+        got = orc.keys(pos, 0.1, M)
+        c = orc.cells(pos, 0.1)
+        want = [interleave(int(a) & 1023, int(b) & 1023, int(d) & 1023) % M for a, b, d in c]
+        np.testing.assert_array_equal(got, np.asarray(want, dtype=np.uint32))
+
+
+@pytest.mark.parametrize("h", [0.1, 1.0, 2.0])
+def test_kernel_constant_normalises_to_pi_over_3_1415(h):
+    """C * integral of D0 over the ball = pi / 3.1415: the reference's constant uses
+    3.1415, not pi (SPHFunctions.cpp:117), and a 'fixed' constant gives exactly 1."""
+    r = np.linspace(0.0, h, 20001)
+    d0 = np.array([orc.kernels(float(x), h)[0] for x in r[::100]] + [0.0])
+    # D0 itself checked against (h - r)^3 on a subsample, then the integral in float64
+    np.testing.assert_allclose(d0[:-1], (h - r[::100]) ** 3, rtol=2e-6, atol=1e-6 * h ** 3)
+    f = 4.0 * math.pi * r ** 2 * (h - r) ** 3
+    integral = float(np.sum((f[1:] + f[:-1]) * 0.5 * np.diff(r)))
+    C = float(orc.kernels(0.0, h)[3])
+    assert abs(C * integral - math.pi / 3.1415) < 2e-6
+    assert abs(C * integral - 1.0) > 2e-5
+
+
+@pytest.mark.parametrize("h", [0.1, 1.0])
+def test_kernel_derivatives(h):
+    """D1 = dD0/dr and D2 = dD1/dr (central differences), all zero for r >= h."""
+    for frac in (0.05, 0.3, 0.5, 0.8, 0.95):
+        r = frac * h
+        dr = 1e-3 * h
+        d0p, d0m = orc.kernels(r + dr, h)[0], orc.kernels(r - dr, h)[0]
+        d1p, d1m = orc.kernels(r + dr, h)[1], orc.kernels(r - dr, h)[1]
+        D0, D1, D2, _ = orc.kernels(r, h)
+        assert D1 == pytest.approx((float(d0p) - float(d0m)) / (2 * dr), rel=2e-3)
+        assert D2 == pytest.approx((float(d1p) - float(d1m)) / (2 * dr), rel=2e-3)
+        assert D0 > 0 and D1 < 0 and D2 > 0
+    for r in (h, h * 1.0001, 2 * h):
+        assert list(orc.kernels(r, h)[:3]) == [0.0, 0.0, 0.0]
+
+
+def test_random_direction_uses_signed_modf():
+    """Noise keeps the sign of sin (C++ modf, SPHFunctions.cpp:143): values lie in
+    (-1, 1) and some are negative (OpenCL's fract would give [0, 1)). Where the
+    noise is in [0, 1) the direction is a unit vector; where it is negative,
+    z = 2 noise - 1 < -1 and the reference's sqrt(1 - z^2) is NaN."""
+    xs = np.arange(0, 400, dtype=np.float32)
+    vals = np.array([orc.noise(float(x)) for x in xs])
+    assert np.all(np.abs(vals) < 1.0)
+    assert np.any(vals < 0) and np.any(vals > 0)
+    for x, v in zip(xs, vals):
+        d = orc.random_direction(float(x))
+        if v >= 0:
+            assert abs(float(np.linalg.norm(d.astype(np.float64))) - 1.0) < 1e-5
+            assert d[2] == pytest.approx(2 * v - 1, abs=1e-6)
+        else:
+            assert np.isnan(d[0]) and np.isnan(d[1])
+
+
+def test_triangle_force_geometry():
+    tri = np.float32([0, 0, 0, 2, 0, 0, 0, 2, 0])  # in the z = 0 plane
+    # above the interior: closest point is the foot of the perpendicular, force along +z
+    for d in (0.5, 1.0, 2.0):
+        f = orc.triangle_force([0.5, 0.5, d], tri)
+        want = 50.0 * (2.5 / d - 1.0) ** 2
+        np.testing.assert_allclose(f, [0, 0, want], rtol=1e-5, atol=1e-6)
+    # beyond the 2.5 range: no force
+    assert list(orc.triangle_force([0.5, 0.5, 2.6], tri)) == [0.0, 0.0, 0.0]
+    # outside vertex a's Voronoi region corner: closest point is the vertex itself
+    f = orc.triangle_force([-1.0, -1.0, 0.0], tri)
+    dist = math.sqrt(2.0)
+    amp = 50.0 * (2.5 / dist - 1.0) ** 2
+    np.testing.assert_allclose(f, [-amp / dist, -amp / dist, 0.0], rtol=1e-5)
+    # beyond the hypotenuse: closest point on edge bc
+    f = orc.triangle_force([2.0, 2.0, 0.0], tri)
+    dist = math.sqrt(2.0)
+    amp = 50.0 * (2.5 / dist - 1.0) ** 2
+    np.testing.assert_allclose(f, [amp / dist, amp / dist, 0.0], rtol=1e-5)
