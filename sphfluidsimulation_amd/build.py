@@ -52,9 +52,22 @@ def build_oracle():
     return os.path.join(ORACLE_DIR, "libsph_oracle.so")
 
 
+ADAPTER_SRC = os.path.join(ROOT, "tests", "cpp", "adapter_demo.cpp")
+ADAPTER_BIN = os.path.join(ROOT, "tests", "cpp", "adapter_demo")
+
+
+def build_adapter_demo():
+    """The C++ SimulationEngineHIP adapter driven like the reference's application
+    (a host-only g++ build against libsph_hip.so)."""
+    _run(["g++", "-std=c++17", "-O2", "-I" + os.path.join(ROOT, "include"), "-o", ADAPTER_BIN, ADAPTER_SRC, LIB,
+          "-Wl,-rpath," + PKG])
+    return ADAPTER_BIN
+
+
 def build_all():
     build_hip()
     build_oracle()
+    build_adapter_demo()
 
 
 if __name__ == "__main__":
