@@ -107,6 +107,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="1m", choices=sorted(CONFIGS))
     ap.add_argument("--graph", action="store_true", help="replay steps from hipGraphs")
+    ap.add_argument("--walk", default="runs", choices=["runs", "cells"],
+                    help="neighbour walk: Morton runs + candidate mask (default) or cell by cell")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--profile-all", action="store_true", help="time every stage with events (default: dominant)")
@@ -120,7 +122,7 @@ def main():
     import numpy as np
 
     from sphfluidsimulation_amd import SimulationEngineHIP, scenes
-    from sphfluidsimulation_amd._lib import SPH_FLAG_HIP_GRAPH
+    from sphfluidsimulation_amd._lib import SPH_FLAG_CELL_WALK, SPH_FLAG_HIP_GRAPH
 
     factory, dt, default_steps = CONFIGS[args.config]
     steps = args.steps if args.steps is not None else default_steps
@@ -130,7 +132,7 @@ def main():
     log(f"[bench] {args.config}: {len(scene_list)} scene(s), "
         f"{sum(s.n for s in scene_list)} particles, generated in {time.perf_counter() - t0:.1f} s")
 
-    flags = SPH_FLAG_HIP_GRAPH if args.graph else 0
+    flags = (SPH_FLAG_HIP_GRAPH if args.graph else 0) | (SPH_FLAG_CELL_WALK if args.walk == "cells" else 0)
     plain = args.config.endswith("-plain")
     engines = []
     if plain:
@@ -209,6 +211,7 @@ def main():
             "launch": "one engine per scene" if plain else "batched" if len(scene_list) > 1 else "single scene",
             "dt": dt,
             "hip_graph": bool(args.graph),
+            "walk": args.walk,
         },
         "roofline": {
             "bound": "hbm",

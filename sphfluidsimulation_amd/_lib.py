@@ -24,6 +24,7 @@ SPH_E_NO_DEVICE = -5
 SPH_FLAG_HIP_GRAPH = 0x1
 SPH_FLAG_RECORD_DEBUG = 0x2
 SPH_FLAG_FP16_OFFSETS = 0x4
+SPH_FLAG_CELL_WALK = 0x8
 
 SPH_V_NONFINITE = 0x1
 SPH_V_KEY = 0x2

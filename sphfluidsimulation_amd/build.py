@@ -64,10 +64,21 @@ def build_adapter_demo():
     return ADAPTER_BIN
 
 
+WALK_SRC = os.path.join(ROOT, "tests", "cpp", "walk_check.cpp")
+WALK_BIN = os.path.join(ROOT, "tests", "cpp", "walk_check")
+
+
+def build_walk_check():
+    """Host-side check of the device neighbour walk (compiled by hipcc, runs on the CPU)."""
+    _run([HIPCC, "-O2", "-std=c++17", "-o", WALK_BIN, WALK_SRC])
+    return WALK_BIN
+
+
 def build_all():
     build_hip()
     build_oracle()
     build_adapter_demo()
+    build_walk_check()
 
 
 if __name__ == "__main__":

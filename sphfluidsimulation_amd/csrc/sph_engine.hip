@@ -64,6 +64,7 @@ struct sph_engine {
     float* tris = nullptr;
     float* dens = nullptr;
     float4* acc = nullptr;
+    uint32_t* mask = nullptr;  // candidate mask from the density pass (run walk)
     double* consPartial = nullptr;
     double* consOut = nullptr;
     uint32_t* valBits = nullptr;
@@ -142,6 +143,7 @@ void release(sph_engine* h) {
     dfree(h->tris);
     dfree(h->dens);
     dfree(h->acc);
+    dfree(h->mask);
     dfree(h->consPartial);
     dfree(h->consOut);
     dfree(h->valBits);
@@ -241,7 +243,18 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     int rc;
     // Density / pressure.
     if ((rc = begin(SPH_STAGE_DENSITY))) return rc;
-    if (h->hasStatic) {
+    const bool runs = !(h->flags & SPH_FLAG_CELL_WALK);
+    if (runs) {
+#define SPH_DENS(S, D)                                                                                         \
+    hipLaunchKernelGGL((k_density_runs<S, D>), grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->pos,      \
+                       h->tab[c], h->spos, h->stab, h->pres, h->mask, h->dens)
+        if (h->hasStatic) {
+            if (dbg) SPH_DENS(true, true); else SPH_DENS(true, false);
+        } else {
+            if (dbg) SPH_DENS(false, true); else SPH_DENS(false, false);
+        }
+#undef SPH_DENS
+    } else if (h->hasStatic) {
         if (dbg)
             hipLaunchKernelGGL((k_density<true, true>), grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->pos,
                                h->tab[c], h->spos, h->stab, h->pres, h->dens);
@@ -262,9 +275,16 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     // Forces, integration, new keys, histogram into the other table.
     if ((rc = begin(SPH_STAGE_FORCE))) return rc;
 #define SPH_FORCE(S, D, T)                                                                                        \
-    hipLaunchKernelGGL((k_force<S, D, T>), grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->pos, h->vel,     \
-                       h->pres, h->tab[c], h->spos, h->stab, h->ids[c], h->tris, dt, h->posT, h->velT, h->tab[c ^ 1], \
-                       h->acc)
+    do {                                                                                                          \
+    if (runs)                                                                                                     \
+        hipLaunchKernelGGL((k_force_runs<S, D, T>), grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->pos,    \
+                           h->vel, h->pres, h->tab[c], h->spos, h->stab, h->mask, h->ids[c], h->tris, dt, h->posT, \
+                           h->velT, h->tab[c ^ 1], h->acc);                                                       \
+    else                                                                                                          \
+        hipLaunchKernelGGL((k_force<S, D, T>), grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->pos, h->vel, \
+                           h->pres, h->tab[c], h->spos, h->stab, h->ids[c], h->tris, dt, h->posT, h->velT,        \
+                           h->tab[c ^ 1], h->acc);                                                                \
+    } while (0)
     if (h->hasStatic) {
         if (dbg) {
             if (h->hasTris) SPH_FORCE(true, true, true); else SPH_FORCE(true, true, false);
@@ -527,6 +547,7 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
     SPH_ALLOC(h->consOut, 8ull * n_scenes);
     SPH_ALLOC(h->valBits, 1);
     SPH_ALLOC(h->valIdx, 8ull * n_scenes);
+    if (!(flags & SPH_FLAG_CELL_WALK)) SPH_ALLOC(h->mask, kMaskWords * std::max<uint64_t>(N, 1));
     if (flags & SPH_FLAG_RECORD_DEBUG) {
         SPH_ALLOC(h->dens, std::max<uint64_t>(N, 1));
         SPH_ALLOC(h->acc, std::max<uint64_t>(N, 1));

@@ -154,6 +154,9 @@ __device__ inline float3 triangle_force(float3 p, const float* t) {
     return make_float3(-d.x / dist * amp, -d.y / dist * amp, -d.z / dist * amp);
 }
 
+constexpr uint32_t kMaskBits = 256;  // candidates per particle recorded by the density pass
+constexpr uint32_t kMaskWords = kMaskBits / 32;
+
 // ---------------------------------------------------------------------------
 // k_density: SF:358-501 for one particle per lane.
 template <bool kStatic, bool kDebug>
@@ -326,6 +329,236 @@ __global__ __launch_bounds__(kBlock) void k_force(const BlockDesc* __restrict__ 
     const float ax = Fx / rho + S.gx, ay = Fy / rho + S.gy, az = Fz / rho + S.gz;
     if (kDebug) accOut[i] = make_float4(ax, ay, az, 0.f);
     // SF:641-645: symplectic Euler, then the key of the new cell.
+    const float nvx = vi.x + ax * dt, nvy = vi.y + ay * dt, nvz = vi.z + az * dt;
+    const float nx = xi.x + nvx * dt, ny = xi.y + nvy * dt, nz = xi.z + nvz * dt;
+    const uint32_t code = morton3(cell_coord(nx, h), cell_coord(ny, h), cell_coord(nz, h));
+    const uint32_t key = fastmod(code, S.magicM, S.M);
+    posT[i] = make_float4(nx, ny, nz, pi);
+    velT[i] = make_float4(nvx, nvy, nvz, __uint_as_float(key));
+    const uint32_t gk = S.tBegin + key;
+    const Run run = wave_run(gk);
+    if (run.is_head) atomicAdd(&counts[gk], run.len);
+}
+
+// ---------------------------------------------------------------------------
+// Run-walk variants of the two neighbour passes. The density pass also records,
+// per particle, which of its first 256 candidates (in walk order) lie within h
+// (a slightly widened test, so the set is a superset of the force pass's exact
+// test). The force pass walks the same runs of the same table and only visits
+// the recorded candidates, plus every candidate past the 256th.
+
+template <bool kStatic, bool kDebug>
+__global__ __launch_bounds__(kBlock) void k_density_runs(const BlockDesc* __restrict__ blocks,
+                                                         const SceneDev* __restrict__ scenes,
+                                                         const float4* __restrict__ pos,
+                                                         const uint32_t* __restrict__ tab,
+                                                         const float4* __restrict__ spos,
+                                                         const uint32_t* __restrict__ stab, float* __restrict__ pres,
+                                                         uint32_t* __restrict__ mask, float* __restrict__ dens) {
+    __shared__ uint32_t smask[kMaskWords * kBlock];
+    const BlockDesc bd = blocks[blockIdx.x];
+    const SceneDev& S = scenes[bd.scene];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t i = bd.begin + tid;
+    if (i >= bd.end) return;
+#pragma unroll
+    for (uint32_t w = 0; w < kMaskWords; ++w) smask[w * kBlock + tid] = 0u;
+    const float4 xi = pos[i];
+    const float h = S.h, h2 = S.h2;
+    const float h2wide = h2 * 1.0001f;
+    const uint32_t cx = cell_coord(xi.x, h), cy = cell_coord(xi.y, h), cz = cell_coord(xi.z, h);
+    float sumDyn = 0.f, sumSt = 0.f;
+    uint32_t k = 0, bits = 0;
+    walk_runs(cx, cy, cz, S.M, S.magicM, tab + S.tBegin, S.pBegin, S.pBegin + S.pCount,
+              [&](uint32_t s, uint32_t e) {
+                  for (uint32_t j = s; j < e; ++j, ++k) {
+                      const float4 xj = pos[j];
+                      const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+                      const float r2 = dx * dx + dy * dy + dz * dz;
+                      if (r2 <= h2wide && j != i) bits |= 1u << (k & 31u);
+                      if ((k & 31u) == 31u) {
+                          if (k < kMaskBits) smask[(k >> 5) * kBlock + tid] = bits;
+                          bits = 0u;
+                      }
+                      if (j == i || r2 > h2) continue;
+                      const float r = __builtin_amdgcn_sqrtf(r2);
+                      const float q = h - r;
+                      sumDyn += (r < h) ? q * q * q : 0.f;
+                  }
+              });
+    if ((k & 31u) != 0u && k < kMaskBits) smask[(k >> 5) * kBlock + tid] = bits;
+    if (kStatic && S.sCount) {
+        walk_runs(cx, cy, cz, S.Ms, S.magicMs, stab + S.stBegin, S.sBegin, S.sBegin + S.sCount,
+                  [&](uint32_t s, uint32_t e) {
+                      for (uint32_t j = s; j < e; ++j) {
+                          const float4 xj = spos[j];
+                          const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+                          const float r2 = dx * dx + dy * dy + dz * dz;
+                          if (r2 > h2) continue;
+                          const float r = __builtin_amdgcn_sqrtf(r2);
+                          const float q = h - r;
+                          sumSt += (r < h) ? q * q * q : 0.f;
+                      }
+                  });
+    }
+    const float rho = S.selfDensity + (sumDyn * S.mass + sumSt * S.mass) * S.C;
+    pres[i] = S.k * (rho - S.rho0);
+    if (kDebug) dens[i] = rho;
+    uint4* mrow = reinterpret_cast<uint4*>(mask + (size_t)i * kMaskWords);
+    mrow[0] = make_uint4(smask[0 * kBlock + tid], smask[1 * kBlock + tid], smask[2 * kBlock + tid],
+                         smask[3 * kBlock + tid]);
+    mrow[1] = make_uint4(smask[4 * kBlock + tid], smask[5 * kBlock + tid], smask[6 * kBlock + tid],
+                         smask[7 * kBlock + tid]);
+}
+
+template <bool kStatic, bool kDebug, bool kTri>
+__global__ __launch_bounds__(kBlock) void k_force_runs(const BlockDesc* __restrict__ blocks,
+                                                       const SceneDev* __restrict__ scenes,
+                                                       const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                                       const float* __restrict__ pres, const uint32_t* __restrict__ tab,
+                                                       const float4* __restrict__ spos,
+                                                       const uint32_t* __restrict__ stab,
+                                                       const uint32_t* __restrict__ mask,
+                                                       const uint32_t* __restrict__ ids, const float* __restrict__ tris,
+                                                       float dt, float4* __restrict__ posT, float4* __restrict__ velT,
+                                                       uint32_t* __restrict__ counts, float4* __restrict__ accOut) {
+    __shared__ uint32_t smask[(kMaskWords + 1) * kBlock];
+    const BlockDesc bd = blocks[blockIdx.x];
+    const SceneDev& S = scenes[bd.scene];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t i = bd.begin + tid;
+    if (i >= bd.end) return;
+    {
+        const uint4* mrow = reinterpret_cast<const uint4*>(mask + (size_t)i * kMaskWords);
+        const uint4 m0 = mrow[0], m1 = mrow[1];
+        smask[0 * kBlock + tid] = m0.x;
+        smask[1 * kBlock + tid] = m0.y;
+        smask[2 * kBlock + tid] = m0.z;
+        smask[3 * kBlock + tid] = m0.w;
+        smask[4 * kBlock + tid] = m1.x;
+        smask[5 * kBlock + tid] = m1.y;
+        smask[6 * kBlock + tid] = m1.z;
+        smask[7 * kBlock + tid] = m1.w;
+        smask[8 * kBlock + tid] = 0u;
+    }
+    const float4 xi = pos[i];
+    const float4 vi = vel[i];
+    const float pi = pres[i];
+    const float h = S.h, h2 = S.h2;
+    const float rho = pi / S.k + S.rho0;  // SF:544
+    const float absP = fabsf(pi);
+    const uint32_t cx = cell_coord(xi.x, h), cy = cell_coord(xi.y, h), cz = cell_coord(xi.z, h);
+    float fpx = 0.f, fpy = 0.f, fpz = 0.f;
+    float fvx = 0.f, fvy = 0.f, fvz = 0.f;
+    float fsx = 0.f, fsy = 0.f, fsz = 0.f;
+    float sumD2s = 0.f;
+    uint32_t k0 = 0;
+    walk_runs(cx, cy, cz, S.M, S.magicM, tab + S.tBegin, S.pBegin, S.pBegin + S.pCount,
+              [&](uint32_t s, uint32_t e) {
+                  const uint32_t n = e > s ? e - s : 0u;
+                  for (uint32_t off = 0; off < n; off += 32u) {
+                      const uint32_t kk = k0 + off;
+                      uint32_t bits;
+                      if (kk + 32u <= kMaskBits) {
+                          const uint32_t w = kk >> 5;
+                          bits = __builtin_amdgcn_alignbit(smask[(w + 1) * kBlock + tid], smask[w * kBlock + tid],
+                                                           kk & 31u);
+                      } else if (kk >= kMaskBits) {
+                          bits = ~0u;  // past the recorded candidates: test them all
+                      } else {
+                          const uint32_t w = kk >> 5;
+                          const uint32_t rec = kMaskBits - kk;  // recorded positions in this chunk (< 32)
+                          const uint32_t m = __builtin_amdgcn_alignbit(smask[(w + 1) * kBlock + tid],
+                                                                       smask[w * kBlock + tid], kk & 31u);
+                          bits = (m & ((1u << rec) - 1u)) | ~((1u << rec) - 1u);
+                      }
+                      if (n - off < 32u) bits &= (1u << (n - off)) - 1u;
+                      while (bits) {
+                          const uint32_t b = (uint32_t)__builtin_ctz(bits);
+                          bits &= bits - 1u;
+                          const uint32_t j = s + off + b;
+                          if (j == i) continue;
+                          const float4 xj = pos[j];
+                          float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+                          const float r2 = dx * dx + dy * dy + dz * dz;
+                          if (r2 > h2) continue;
+                          const float r = __builtin_amdgcn_sqrtf(r2);
+                          if (r2 == 0.f || r == 0.f) {
+                              const float3 d = random_direction((float)(ids[i] - S.pBegin));
+                              dx = d.x;
+                              dy = d.y;
+                              dz = d.z;
+                          } else {
+                              const float inv = __builtin_amdgcn_rcpf(r);
+                              dx *= inv;
+                              dy *= inv;
+                              dz *= inv;
+                          }
+                          const float q = h - r;
+                          const bool in = r < h;
+                          const float w1 = in ? -3.0f * q * q : 0.f;
+                          const float w2 = in ? 6.0f * q : 0.f;
+                          const float pj = pres[j];
+                          const float4 vj = vel[j];
+                          const float sp = (pi + pj) * w1;
+                          fpx += dx * sp;
+                          fpy += dy * sp;
+                          fpz += dz * sp;
+                          fvx += (vj.x - vi.x) * w2;
+                          fvy += (vj.y - vi.y) * w2;
+                          fvz += (vj.z - vi.z) * w2;
+                      }
+                  }
+                  k0 += n;
+              });
+    if (kStatic && S.sCount) {
+        walk_runs(cx, cy, cz, S.Ms, S.magicMs, stab + S.stBegin, S.sBegin, S.sBegin + S.sCount,
+                  [&](uint32_t s, uint32_t e) {
+                      for (uint32_t j = s; j < e; ++j) {
+                          const float4 xj = spos[j];
+                          float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+                          const float r2 = dx * dx + dy * dy + dz * dz;
+                          if (r2 > h2) continue;
+                          const float r = __builtin_amdgcn_sqrtf(r2);
+                          if (r2 == 0.f || r == 0.f) {
+                              const float3 d = random_direction((float)(ids[i] - S.pBegin));
+                              dx = d.x;
+                              dy = d.y;
+                              dz = d.z;
+                          } else {
+                              const float inv = __builtin_amdgcn_rcpf(r);
+                              dx *= inv;
+                              dy *= inv;
+                              dz *= inv;
+                          }
+                          const float q = h - r;
+                          const bool in = r < h;
+                          const float sp = absP * (in ? -3.0f * q * q : 0.f);
+                          fsx += dx * sp;
+                          fsy += dy * sp;
+                          fsz += dz * sp;
+                          sumD2s += in ? 6.0f * q : 0.f;
+                      }
+                  });
+    }
+    // SF:626-638
+    const float sP = S.mass / (2.0f * rho) * S.C;
+    const float sV = S.mu * S.mass * S.C;
+    const float sVs = S.mu * S.mass * S.C * 0.0f;
+    float Fx = fpx * sP + fvx * sV + fsx * sP + (-vi.x * sumD2s) * sVs;
+    float Fy = fpy * sP + fvy * sV + fsy * sP + (-vi.y * sumD2s) * sVs;
+    float Fz = fpz * sP + fvz * sV + fsz * sP + (-vi.z * sumD2s) * sVs;
+    if (kTri) {
+        const float3 p3 = make_float3(xi.x, xi.y, xi.z);
+        for (uint32_t t = 0; t < S.triCount; ++t) {
+            const float3 f = triangle_force(p3, tris + 9u * (S.triBegin + t));
+            Fx += f.x;
+            Fy += f.y;
+            Fz += f.z;
+        }
+    }
+    const float ax = Fx / rho + S.gx, ay = Fy / rho + S.gy, az = Fz / rho + S.gz;
+    if (kDebug) accOut[i] = make_float4(ax, ay, az, 0.f);
     const float nvx = vi.x + ax * dt, nvy = vi.y + ay * dt, nvz = vi.z + az * dt;
     const float nx = xi.x + nvx * dt, ny = xi.y + nvy * dt, nz = xi.z + nvz * dt;
     const uint32_t code = morton3(cell_coord(nx, h), cell_coord(ny, h), cell_coord(nz, h));

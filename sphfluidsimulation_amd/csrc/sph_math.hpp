@@ -36,9 +36,13 @@ __host__ __device__ inline uint32_t morton3(uint32_t cx, uint32_t cy, uint32_t c
 // numerator and divisor. magic = floor((2^64 - 1) / d) + 1 (0 for d = 1).
 __host__ __device__ inline uint64_t fastmod_magic(uint32_t d) { return UINT64_MAX / d + 1u; }
 
-__device__ inline uint32_t fastmod(uint32_t a, uint64_t magic, uint32_t d) {
+// The high 64 bits of low * d are assembled from two 32x32->64 products, so the
+// same code runs on the host (the initial sort and the tests) and the device.
+__host__ __device__ inline uint32_t fastmod(uint32_t a, uint64_t magic, uint32_t d) {
     const uint64_t low = magic * a;
-    return (uint32_t)__umul64hi(low, (uint64_t)d);
+    const uint64_t t = (uint64_t)(uint32_t)low * d;
+    const uint64_t u = (low >> 32) * d + (t >> 32);
+    return (uint32_t)(u >> 32);
 }
 
 // SF:115-118: the reference's constant uses 3.1415f, not pi, and multiplies h in
@@ -49,5 +53,77 @@ __host__ __device__ inline float kernel_constant(float h) {
 
 // SF:119-125 at r = 0, for selfDensity = m * D0(0, h) * C (SimulationEngineCPU.cpp:407).
 __host__ __device__ inline float d0_at_zero(float h) { return (h - 0.0f) * (h - 0.0f) * (h - 0.0f); }
+
+// ---------------------------------------------------------------------------
+// Neighbour walk by Morton runs.
+//
+// The reference visits the 27 cells around a particle one by one (SF:397-404)
+// and scans each cell's bucket. Per axis the three cells c-1, c, c+1 split into
+// an even-aligned pair {c & ~1, (c & ~1) + 1} and a single cell, and the Morton
+// codes of an aligned pair differ only in that axis's lowest bit. So the 27
+// cells form 14 runs of consecutive codes: pair x pair x pair (8 codes),
+// pair x pair x single (4), pair x single x pair (two runs of 2: z adds 4),
+// single x pair x pair (four runs of 1: y adds 2, z adds 4), pair x single x
+// single (2), single x pair x single (two of 1), single x single x pair (two of
+// 1), single x single x single (1). A run of L consecutive codes starting at an
+// L-aligned code maps to L consecutive buckets b0 .. b0+L-1 unless it wraps
+// past M, and consecutive buckets are one contiguous particle range
+// [table[b0], table[b0+L]). Wrapping runs are walked bucket by bucket, so every
+// bucket is visited exactly as often as the reference visits it (colliding
+// cells included).
+struct CellPair {
+    uint32_t p, s;  // spread3 of the aligned pair base and of the single cell, pre-shifted
+};
+
+__host__ __device__ inline CellPair axis_pair(uint32_t c, uint32_t shift) {
+    CellPair r;
+    r.p = spread3(c & ~1u) << shift;
+    r.s = spread3((c & 1u) ? c + 1u : c - 1u) << shift;
+    return r;
+}
+
+// Run r (0..13): first Morton code and length in codes.
+__host__ __device__ inline void run_of(uint32_t r, const CellPair& X, const CellPair& Y, const CellPair& Z,
+                                       uint32_t& code, uint32_t& len) {
+    switch (r) {
+        case 0: code = X.p | Y.p | Z.p; len = 8; break;
+        case 1: code = X.p | Y.p | Z.s; len = 4; break;
+        case 2: code = X.p | Y.s | Z.p; len = 2; break;
+        case 3: code = (X.p | Y.s | Z.p) + 4u; len = 2; break;
+        case 4: code = X.s | Y.p | Z.p; len = 1; break;
+        case 5: code = (X.s | Y.p | Z.p) + 2u; len = 1; break;
+        case 6: code = (X.s | Y.p | Z.p) + 4u; len = 1; break;
+        case 7: code = (X.s | Y.p | Z.p) + 6u; len = 1; break;
+        case 8: code = X.p | Y.s | Z.s; len = 2; break;
+        case 9: code = X.s | Y.p | Z.s; len = 1; break;
+        case 10: code = (X.s | Y.p | Z.s) + 2u; len = 1; break;
+        case 11: code = X.s | Y.s | Z.p; len = 1; break;
+        case 12: code = (X.s | Y.s | Z.p) + 4u; len = 1; break;
+        default: code = X.s | Y.s | Z.s; len = 1; break;
+    }
+}
+
+// Calls f(begin, end) for every particle range the 27-cell walk visits, in a
+// fixed order. `table` is the scene's table segment (M + 1 global indices);
+// ranges are clamped to [lo, hi) so a corrupted table cannot read outside the
+// particle arrays.
+template <class F>
+__host__ __device__ inline void walk_runs(uint32_t cx, uint32_t cy, uint32_t cz, uint32_t M, uint64_t magic,
+                                          const uint32_t* __restrict__ table, uint32_t lo, uint32_t hi, F&& f) {
+    const CellPair X = axis_pair(cx, 0), Y = axis_pair(cy, 1), Z = axis_pair(cz, 2);
+    for (uint32_t r = 0; r < 14; ++r) {
+        uint32_t code, len;
+        run_of(r, X, Y, Z, code, len);
+        const uint32_t b0 = fastmod(code, magic, M);
+        const bool wraps = b0 + len > M;
+        const uint32_t parts = wraps ? len : 1u;
+        for (uint32_t part = 0; part < parts; ++part) {
+            const uint32_t b = wraps ? fastmod(b0 + part, magic, M) : b0;
+            const uint32_t bl = wraps ? 1u : len;
+            const uint32_t tb = table[b], te = table[b + bl];
+            f(tb > lo ? tb : lo, te < hi ? te : hi);
+        }
+    }
+}
 
 }  // namespace sph

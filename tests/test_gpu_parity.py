@@ -19,13 +19,17 @@ import pytest
 import oracle as orc
 from helpers import bucket_multisets_equal, check_buckets, expected_starts, max_norm_rel, max_rel
 from sphfluidsimulation_amd import SPHError, SimulationEngineHIP, scenes
-from sphfluidsimulation_amd._lib import SPH_E_STATE, SPH_FLAG_HIP_GRAPH, SPH_FLAG_RECORD_DEBUG
+from sphfluidsimulation_amd._lib import SPH_E_STATE, SPH_FLAG_CELL_WALK, SPH_FLAG_HIP_GRAPH, SPH_FLAG_RECORD_DEBUG
 
 pytestmark = pytest.mark.gpu
 
 DENS_TOL = 1e-5
 ACC_TOL = 1e-5
 FP64_DENS_TOL = 1e-4
+
+# The two neighbour walks: Morton runs with the candidate mask (default) and the
+# reference's cell-by-cell loop.
+WALKS = [pytest.param(0, id="runs"), pytest.param(SPH_FLAG_CELL_WALK, id="cells")]
 
 
 def say(msg):
@@ -67,9 +71,9 @@ def _init_parity(sc, threads=1):
     ref.close()
 
 
-def _one_step(sc, dt, threads):
+def _one_step(sc, dt, threads, walk=0):
     e = SimulationEngineHIP(0)
-    e.Initialize(sc, flags=SPH_FLAG_RECORD_DEBUG)
+    e.Initialize(sc, flags=SPH_FLAG_RECORD_DEBUG | walk)
     ref = orc.Oracle(sc.params, sc.pos, sc.vel, sc.static_pos, threads=threads, flags=orc.FLAG_RECORD_DEBUG)
     e.Update(dt, 1)
     ref.update(dt, 1)
@@ -124,9 +128,10 @@ def test_init_parity_16k(s16k):
     _init_parity(s16k)
 
 
-def test_one_step_16k(s16k):
-    say("one step 16k")
-    got, ref = _one_step(s16k, 1e-3, threads=4)
+@pytest.mark.parametrize("walk", WALKS)
+def test_one_step_16k(s16k, walk):
+    say(f"one step 16k (walk flags {walk})")
+    got, ref = _one_step(s16k, 1e-3, threads=4, walk=walk)
     _check_step(s16k, got, ref, 1e-3)
 
 
@@ -149,10 +154,11 @@ def test_density_vs_fp64_16k(s16k):
     ref.close()
 
 
-def test_step_integer_parity_16k(s16k):
-    say("integer parity after 5 steps 16k")
+@pytest.mark.parametrize("walk", WALKS)
+def test_step_integer_parity_16k(s16k, walk):
+    say(f"integer parity after 5 steps 16k (walk flags {walk})")
     e = SimulationEngineHIP(0)
-    e.Initialize(s16k)
+    e.Initialize(s16k, flags=walk)
     e.Update(1e-3, 5)
     _step_integer_parity(e, s16k)
     e.close()
@@ -163,9 +169,10 @@ def test_init_parity_1m(s1m):
     _init_parity(s1m, threads=8)
 
 
-def test_one_step_1m(s1m):
-    say("one step 1M")
-    got, ref = _one_step(s1m, 1e-3, threads=16)
+@pytest.mark.parametrize("walk", WALKS)
+def test_one_step_1m(s1m, walk):
+    say(f"one step 1M (walk flags {walk})")
+    got, ref = _one_step(s1m, 1e-3, threads=16, walk=walk)
     _check_step(s1m, got, ref, 1e-3)
 
 
@@ -248,20 +255,22 @@ def test_batched_against_oracle():
     e.close()
 
 
-def test_negative_coordinates_wrap(s16k):
-    say("negative coordinates (uint wrap of cells)")
+@pytest.mark.parametrize("walk", WALKS)
+def test_negative_coordinates_wrap(s16k, walk):
+    say(f"negative coordinates (uint wrap of cells; walk flags {walk})")
     sc = scenes.Scene(s16k.params, (s16k.pos - np.float32([2.05, 1.0, 1.5])).astype(np.float32), s16k.vel,
                       (s16k.static_pos - np.float32([2.05, 1.0, 1.5])).astype(np.float32), name="shifted-16k")
     _init_parity(sc)
-    got, ref = _one_step(sc, 1e-3, threads=4)
+    got, ref = _one_step(sc, 1e-3, threads=4, walk=walk)
     _check_step(sc, got, ref, 1e-3)
 
 
-def test_debug_smallest_scene():
-    say("debugSmallestScene (h = 1, M = 16, colliding buckets)")
+@pytest.mark.parametrize("walk", WALKS)
+def test_debug_smallest_scene(walk):
+    say(f"debugSmallestScene (h = 1, M = 16, colliding buckets; walk flags {walk})")
     sc = scenes.debug_smallest()
     _init_parity(sc)
-    got, ref = _one_step(sc, 1e-2, threads=1)
+    got, ref = _one_step(sc, 1e-2, threads=1, walk=walk)
     _check_step(sc, got, ref, 1e-2)
 
 
