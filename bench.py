@@ -112,6 +112,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--profile-all", action="store_true", help="time every stage with events (default: dominant)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES (HBM counter calibration) and exit")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -119,10 +121,16 @@ def main():
               "(multi-GPU sharding is a design note, not an implementation)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
 
     from sphfluidsimulation_amd import SimulationEngineHIP, scenes
     from sphfluidsimulation_amd._lib import SPH_FLAG_CELL_WALK, SPH_FLAG_HIP_GRAPH
+
+    if args.copy_probe:
+        e = SimulationEngineHIP(0)
+        e.copy_probe(args.copy_probe, 3)
+        e.close()
+        log(f"[bench] copy probe: 3 copies of {args.copy_probe} bytes")
+        return
 
     factory, dt, default_steps = CONFIGS[args.config]
     steps = args.steps if args.steps is not None else default_steps

@@ -190,6 +190,12 @@ uint64_t sph_hash_map_size(sph_handle h, uint32_t scene);
 int sph_set_profiling(sph_handle h, uint32_t stage_mask);
 int sph_stage_times(sph_handle h, double* ms, uint64_t* launches);
 
+/* Measurement aid with no counterpart in the reference: launches `repeats` plain
+ * copies of `bytes` bytes between two device buffers (kernel k_copy_probe), so
+ * that HBM byte counters read under a profiler can be scaled against a kernel
+ * whose traffic is known exactly. Synchronous. */
+int sph_copy_probe(sph_handle h, uint64_t bytes, uint32_t repeats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -85,6 +85,7 @@ SIGNATURES = {
     "sph_hash_map_size": (ctypes.c_uint64, [_P, ctypes.c_uint32]),
     "sph_set_profiling": (ctypes.c_int, [_P, ctypes.c_uint32]),
     "sph_stage_times": (ctypes.c_int, [_P, _F64P, _U64P]),
+    "sph_copy_probe": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint32]),
 }
 
 _lib = None

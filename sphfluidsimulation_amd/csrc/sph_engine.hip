@@ -901,4 +901,30 @@ int sph_stage_times(sph_handle h, double* ms, uint64_t* launches) {
     return SPH_OK;
 }
 
+int sph_copy_probe(sph_handle h, uint64_t bytes, uint32_t repeats) {
+    if (!valid(h) || bytes < 16) return SPH_E_INVALID;
+    SPH_HIP(h, hipSetDevice(h->device));
+    const uint64_t n16 = bytes / 16;
+    uint4 *a = nullptr, *b = nullptr;
+    int rc;
+    if ((rc = dalloc(h, &a, n16)) || (rc = dalloc(h, &b, n16))) {
+        dfree(a);
+        dfree(b);
+        return rc;
+    }
+    hipError_t e = hipMemsetAsync(a, 1, n16 * 16, h->stream);
+    for (uint32_t r = 0; r < repeats && e == hipSuccess; ++r) {
+        hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, h->stream, a, b, n16);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    dfree(a);
+    dfree(b);
+    if (e != hipSuccess) {
+        h->err = std::string("sph_copy_probe: ") + hipGetErrorString(e);
+        return SPH_E_HIP;
+    }
+    return SPH_OK;
+}
+
 }  // extern "C"

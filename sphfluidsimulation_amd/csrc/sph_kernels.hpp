@@ -905,4 +905,11 @@ __global__ __launch_bounds__(kBlock) void k_unsort_debug(const float* __restrict
     outA[3ull * o + 2] = a.z;
 }
 
+// Plain streaming copy of n16 16-byte words (counter calibration).
+__global__ __launch_bounds__(256) void k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                    uint64_t n16) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n16; k += stride) dst[k] = src[k];
+}
+
 }  // namespace sph

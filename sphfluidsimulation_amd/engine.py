@@ -201,3 +201,7 @@ class SimulationEngineHIP:
         n = np.zeros(4, np.uint64)
         self._check(self._lib.sph_stage_times(self._h, _ptr(ms, ctypes.c_double), _ptr(n, ctypes.c_uint64)))
         return {name: (float(ms[i]), int(n[i])) for i, name in enumerate(_lib.STAGES)}
+
+    def copy_probe(self, nbytes: int, repeats: int = 1):
+        """Plain device copy of nbytes (counter calibration under a profiler)."""
+        self._check(self._lib.sph_copy_probe(self._h, int(nbytes), int(repeats)))
