@@ -219,6 +219,7 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     const bool dbg = (h->flags & SPH_FLAG_RECORD_DEBUG) != 0;
     const uint32_t prof = capture ? 0u : h->profiling;
     const dim3 grid(h->numBlocks), block(kBlock);
+    const dim3 gridX(((h->numBlocks + 7u) / 8u) * 8u);  // padded for the XCD-aware tile order
     StageEvent ev{};
     bool timing = false;
     auto begin = [&](int stage) -> int {
@@ -246,8 +247,8 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     const bool runs = !(h->flags & SPH_FLAG_CELL_WALK);
     if (runs) {
 #define SPH_DENS(S, D)                                                                                         \
-    hipLaunchKernelGGL((k_density_runs<S, D>), grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->pos,      \
-                       h->tab[c], h->spos, h->stab, h->pres, h->mask, h->dens)
+    hipLaunchKernelGGL((k_density_runs<S, D>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks, h->dScenes, \
+                       h->pos, h->tab[c], h->spos, h->stab, h->mask, h->dens)
         if (h->hasStatic) {
             if (dbg) SPH_DENS(true, true); else SPH_DENS(true, false);
         } else {
@@ -277,9 +278,9 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
 #define SPH_FORCE(S, D, T)                                                                                        \
     do {                                                                                                          \
     if (runs)                                                                                                     \
-        hipLaunchKernelGGL((k_force_runs<S, D, T>), grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->pos,    \
-                           h->vel, h->pres, h->tab[c], h->spos, h->stab, h->mask, h->ids[c], h->tris, dt, h->posT, \
-                           h->velT, h->tab[c ^ 1], h->acc);                                                       \
+        hipLaunchKernelGGL((k_force_runs<S, D, T>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,        \
+                           h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->mask, h->ids[c], h->tris,  \
+                           dt, h->posT, h->velT, h->tab[c ^ 1], h->acc);                                          \
     else                                                                                                          \
         hipLaunchKernelGGL((k_force<S, D, T>), grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->pos, h->vel, \
                            h->pres, h->tab[c], h->spos, h->stab, h->ids[c], h->tris, dt, h->posT, h->velT,        \

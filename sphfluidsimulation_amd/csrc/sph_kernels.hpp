@@ -345,18 +345,30 @@ __global__ __launch_bounds__(kBlock) void k_force(const BlockDesc* __restrict__ 
 // per particle, which of its first 256 candidates (in walk order) lie within h
 // (a slightly widened test, so the set is a superset of the force pass's exact
 // test). The force pass walks the same runs of the same table and only visits
-// the recorded candidates, plus every candidate past the 256th.
+// the recorded candidates, plus every candidate past the 256th. The density
+// pass stores the pressure in pos.w, so the force pass gets a neighbour's
+// position and pressure from one 16-byte load.
+
+// Workgroups are dispatched round-robin over the 8 XCDs (launch index b runs
+// on XCD b % 8). The launch is padded to a multiple of 8 and index b takes
+// particle tile (b % 8) * chunk + b / 8, so each XCD walks one contiguous
+// eighth of the sorted particles and neighbouring tiles share that XCD's L2.
+__device__ inline uint32_t xcd_tile(uint32_t b, uint32_t numTiles) {
+    const uint32_t chunk = (numTiles + 7u) >> 3;
+    return (b & 7u) * chunk + (b >> 3);
+}
 
 template <bool kStatic, bool kDebug>
-__global__ __launch_bounds__(kBlock) void k_density_runs(const BlockDesc* __restrict__ blocks,
-                                                         const SceneDev* __restrict__ scenes,
-                                                         const float4* __restrict__ pos,
+__global__ __launch_bounds__(kBlock) void k_density_runs(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
+                                                         const SceneDev* __restrict__ scenes, float4* pos,
                                                          const uint32_t* __restrict__ tab,
                                                          const float4* __restrict__ spos,
-                                                         const uint32_t* __restrict__ stab, float* __restrict__ pres,
+                                                         const uint32_t* __restrict__ stab,
                                                          uint32_t* __restrict__ mask, float* __restrict__ dens) {
     __shared__ uint32_t smask[kMaskWords * kBlock];
-    const BlockDesc bd = blocks[blockIdx.x];
+    const uint32_t tile = xcd_tile(blockIdx.x, numBlocks);
+    if (tile >= numBlocks) return;
+    const BlockDesc bd = blocks[tile];
     const SceneDev& S = scenes[bd.scene];
     const uint32_t tid = threadIdx.x;
     const uint32_t i = bd.begin + tid;
@@ -371,19 +383,30 @@ __global__ __launch_bounds__(kBlock) void k_density_runs(const BlockDesc* __rest
     uint32_t k = 0, bits = 0;
     walk_runs(cx, cy, cz, S.M, S.magicM, tab + S.tBegin, S.pBegin, S.pBegin + S.pCount,
               [&](uint32_t s, uint32_t e) {
-                  for (uint32_t j = s; j < e; ++j, ++k) {
-                      const float4 xj = pos[j];
-                      const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-                      const float r2 = dx * dx + dy * dy + dz * dz;
-                      if (r2 <= h2wide && j != i) bits |= 1u << (k & 31u);
-                      if ((k & 31u) == 31u) {
-                          if (k < kMaskBits) smask[(k >> 5) * kBlock + tid] = bits;
-                          bits = 0u;
+                  // Four candidates per iteration, loads issued before use.
+                  for (uint32_t j = s; j < e; j += 4u) {
+                      const uint32_t last = e - 1u;
+                      float4 xs[4];
+#pragma unroll
+                      for (uint32_t q = 0; q < 4; ++q) xs[q] = pos[min(j + q, last)];
+#pragma unroll
+                      for (uint32_t q = 0; q < 4; ++q) {
+                          if (j + q >= e) break;
+                          const float dx = xs[q].x - xi.x, dy = xs[q].y - xi.y, dz = xs[q].z - xi.z;
+                          const float r2 = dx * dx + dy * dy + dz * dz;
+                          const bool self = j + q == i;
+                          if (r2 <= h2wide && !self) bits |= 1u << (k & 31u);
+                          if ((k & 31u) == 31u) {
+                              if (k < kMaskBits) smask[(k >> 5) * kBlock + tid] = bits;
+                              bits = 0u;
+                          }
+                          ++k;
+                          if (!self && r2 <= h2) {
+                              const float r = __builtin_amdgcn_sqrtf(r2);
+                              const float qh = h - r;
+                              sumDyn += (r < h) ? qh * qh * qh : 0.f;
+                          }
                       }
-                      if (j == i || r2 > h2) continue;
-                      const float r = __builtin_amdgcn_sqrtf(r2);
-                      const float q = h - r;
-                      sumDyn += (r < h) ? q * q * q : 0.f;
                   }
               });
     if ((k & 31u) != 0u && k < kMaskBits) smask[(k >> 5) * kBlock + tid] = bits;
@@ -402,7 +425,8 @@ __global__ __launch_bounds__(kBlock) void k_density_runs(const BlockDesc* __rest
                   });
     }
     const float rho = S.selfDensity + (sumDyn * S.mass + sumSt * S.mass) * S.C;
-    pres[i] = S.k * (rho - S.rho0);
+    // Only this lane writes pos[i].w, and no lane of this pass reads any .w.
+    reinterpret_cast<float*>(pos)[4ull * i + 3] = S.k * (rho - S.rho0);
     if (kDebug) dens[i] = rho;
     uint4* mrow = reinterpret_cast<uint4*>(mask + (size_t)i * kMaskWords);
     mrow[0] = make_uint4(smask[0 * kBlock + tid], smask[1 * kBlock + tid], smask[2 * kBlock + tid],
@@ -412,10 +436,10 @@ __global__ __launch_bounds__(kBlock) void k_density_runs(const BlockDesc* __rest
 }
 
 template <bool kStatic, bool kDebug, bool kTri>
-__global__ __launch_bounds__(kBlock) void k_force_runs(const BlockDesc* __restrict__ blocks,
+__global__ __launch_bounds__(kBlock) void k_force_runs(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
                                                        const SceneDev* __restrict__ scenes,
                                                        const float4* __restrict__ pos, const float4* __restrict__ vel,
-                                                       const float* __restrict__ pres, const uint32_t* __restrict__ tab,
+                                                       const uint32_t* __restrict__ tab,
                                                        const float4* __restrict__ spos,
                                                        const uint32_t* __restrict__ stab,
                                                        const uint32_t* __restrict__ mask,
@@ -423,7 +447,9 @@ __global__ __launch_bounds__(kBlock) void k_force_runs(const BlockDesc* __restri
                                                        float dt, float4* __restrict__ posT, float4* __restrict__ velT,
                                                        uint32_t* __restrict__ counts, float4* __restrict__ accOut) {
     __shared__ uint32_t smask[(kMaskWords + 1) * kBlock];
-    const BlockDesc bd = blocks[blockIdx.x];
+    const uint32_t tile = xcd_tile(blockIdx.x, numBlocks);
+    if (tile >= numBlocks) return;
+    const BlockDesc bd = blocks[tile];
     const SceneDev& S = scenes[bd.scene];
     const uint32_t tid = threadIdx.x;
     const uint32_t i = bd.begin + tid;
@@ -443,7 +469,7 @@ __global__ __launch_bounds__(kBlock) void k_force_runs(const BlockDesc* __restri
     }
     const float4 xi = pos[i];
     const float4 vi = vel[i];
-    const float pi = pres[i];
+    const float pi = xi.w;  // this step's pressure (density pass)
     const float h = S.h, h2 = S.h2;
     const float rho = pi / S.k + S.rho0;  // SF:544
     const float absP = fabsf(pi);
@@ -452,6 +478,35 @@ __global__ __launch_bounds__(kBlock) void k_force_runs(const BlockDesc* __restri
     float fvx = 0.f, fvy = 0.f, fvz = 0.f;
     float fsx = 0.f, fsy = 0.f, fsz = 0.f;
     float sumD2s = 0.f;
+    // One dynamic neighbour j (SF:557-589).
+    auto pair = [&](uint32_t j, const float4& xj, const float4& vj) {
+        float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (j == i || r2 > h2) return;
+        const float r = __builtin_amdgcn_sqrtf(r2);
+        if (r2 == 0.f || r == 0.f) {
+            const float3 d = random_direction((float)(ids[i] - S.pBegin));
+            dx = d.x;
+            dy = d.y;
+            dz = d.z;
+        } else {
+            const float inv = __builtin_amdgcn_rcpf(r);
+            dx *= inv;
+            dy *= inv;
+            dz *= inv;
+        }
+        const float q = h - r;
+        const bool in = r < h;
+        const float w1 = in ? -3.0f * q * q : 0.f;
+        const float w2 = in ? 6.0f * q : 0.f;
+        const float sp = (pi + xj.w) * w1;
+        fpx += dx * sp;
+        fpy += dy * sp;
+        fpz += dz * sp;
+        fvx += (vj.x - vi.x) * w2;
+        fvy += (vj.y - vi.y) * w2;
+        fvz += (vj.z - vi.z) * w2;
+    };
     uint32_t k0 = 0;
     walk_runs(cx, cy, cz, S.M, S.magicM, tab + S.tBegin, S.pBegin, S.pBegin + S.pCount,
               [&](uint32_t s, uint32_t e) {
@@ -473,40 +528,18 @@ __global__ __launch_bounds__(kBlock) void k_force_runs(const BlockDesc* __restri
                           bits = (m & ((1u << rec) - 1u)) | ~((1u << rec) - 1u);
                       }
                       if (n - off < 32u) bits &= (1u << (n - off)) - 1u;
+                      // Two neighbours per iteration, all four loads issued before use.
                       while (bits) {
-                          const uint32_t b = (uint32_t)__builtin_ctz(bits);
+                          const uint32_t b0 = (uint32_t)__builtin_ctz(bits);
                           bits &= bits - 1u;
-                          const uint32_t j = s + off + b;
-                          if (j == i) continue;
-                          const float4 xj = pos[j];
-                          float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-                          const float r2 = dx * dx + dy * dy + dz * dz;
-                          if (r2 > h2) continue;
-                          const float r = __builtin_amdgcn_sqrtf(r2);
-                          if (r2 == 0.f || r == 0.f) {
-                              const float3 d = random_direction((float)(ids[i] - S.pBegin));
-                              dx = d.x;
-                              dy = d.y;
-                              dz = d.z;
-                          } else {
-                              const float inv = __builtin_amdgcn_rcpf(r);
-                              dx *= inv;
-                              dy *= inv;
-                              dz *= inv;
-                          }
-                          const float q = h - r;
-                          const bool in = r < h;
-                          const float w1 = in ? -3.0f * q * q : 0.f;
-                          const float w2 = in ? 6.0f * q : 0.f;
-                          const float pj = pres[j];
-                          const float4 vj = vel[j];
-                          const float sp = (pi + pj) * w1;
-                          fpx += dx * sp;
-                          fpy += dy * sp;
-                          fpz += dz * sp;
-                          fvx += (vj.x - vi.x) * w2;
-                          fvy += (vj.y - vi.y) * w2;
-                          fvz += (vj.z - vi.z) * w2;
+                          const bool two = bits != 0u;
+                          const uint32_t b1 = two ? (uint32_t)__builtin_ctz(bits) : b0;
+                          if (two) bits &= bits - 1u;
+                          const uint32_t j0 = s + off + b0, j1 = s + off + b1;
+                          const float4 x0 = pos[j0], v0 = vel[j0];
+                          const float4 x1 = pos[j1], v1 = vel[j1];
+                          pair(j0, x0, v0);
+                          if (two) pair(j1, x1, v1);
                       }
                   }
                   k0 += n;
