@@ -42,8 +42,10 @@ extern "C" {
 #define SPH_FLAG_HIP_GRAPH 0x1u     /* replay each step from a captured hipGraph */
 #define SPH_FLAG_RECORD_DEBUG 0x2u  /* keep per-particle density and acceleration of the last step */
 #define SPH_FLAG_FP16_OFFSETS 0x4u  /* mixed precision: FP16 cell-centre offsets, Kahan density sums */
-#define SPH_FLAG_CELL_WALK 0x8u     /* walk the 27 neighbour cells one by one (no run merging, no
-                                       candidate mask): the reference's loop shape, kept for comparison */
+#define SPH_FLAG_CELL_WALK 0x8u     /* neighbour passes read every candidate from HBM, one cell at a
+                                       time: the reference's loop shape, kept for comparison */
+#define SPH_FLAG_RUN_WALK 0x10u     /* neighbour passes read candidates from HBM in merged Morton runs;
+                                       default (neither flag): LDS-staged bucket tiles */
 
 /* sph_validate result bits (the invariants of SimulationEngine.h:115-282). */
 #define SPH_V_NONFINITE 0x1u        /* NaN or Inf in position, velocity or pressure */

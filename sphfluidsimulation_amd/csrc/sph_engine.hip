@@ -17,6 +17,7 @@
 
 #include "../../include/sph_hip.h"
 #include "sph_kernels.hpp"
+#include "sph_tile.hpp"
 
 using namespace sph;
 
@@ -244,15 +245,29 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     int rc;
     // Density / pressure.
     if ((rc = begin(SPH_STAGE_DENSITY))) return rc;
-    const bool runs = !(h->flags & SPH_FLAG_CELL_WALK);
-    if (runs) {
+    const bool cells = (h->flags & SPH_FLAG_CELL_WALK) != 0;
+    const bool runs = !cells && (h->flags & SPH_FLAG_RUN_WALK) != 0;
+    const bool tiles = !cells && !runs;
+    if (!cells) {
 #define SPH_DENS(S, D)                                                                                         \
-    hipLaunchKernelGGL((k_density_runs<S, D>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks, h->dScenes, \
-                       h->pos, h->tab[c], h->spos, h->stab, h->mask, h->dens)
+    if (tiles)                                                                                                 \
+        hipLaunchKernelGGL((k_density_tile<S, D>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,       \
+                           h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->mask, h->dens);                 \
+    else                                                                                                       \
+        hipLaunchKernelGGL((k_density_runs<S, D>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,       \
+                           h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->mask, h->dens)
         if (h->hasStatic) {
-            if (dbg) SPH_DENS(true, true); else SPH_DENS(true, false);
+            if (dbg) {
+                SPH_DENS(true, true);
+            } else {
+                SPH_DENS(true, false);
+            }
         } else {
-            if (dbg) SPH_DENS(false, true); else SPH_DENS(false, false);
+            if (dbg) {
+                SPH_DENS(false, true);
+            } else {
+                SPH_DENS(false, false);
+            }
         }
 #undef SPH_DENS
     } else if (h->hasStatic) {
@@ -277,7 +292,11 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     if ((rc = begin(SPH_STAGE_FORCE))) return rc;
 #define SPH_FORCE(S, D, T)                                                                                        \
     do {                                                                                                          \
-    if (runs)                                                                                                     \
+    if (tiles)                                                                                                    \
+        hipLaunchKernelGGL((k_force_tile<S, D, T>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,         \
+                           h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->mask, h->ids[c], h->tris,  \
+                           dt, h->posT, h->velT, h->tab[c ^ 1], h->acc);                                          \
+    else if (runs)                                                                                                \
         hipLaunchKernelGGL((k_force_runs<S, D, T>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,        \
                            h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->mask, h->ids[c], h->tris,  \
                            dt, h->posT, h->velT, h->tab[c ^ 1], h->acc);                                          \
@@ -548,7 +567,7 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
     SPH_ALLOC(h->consOut, 8ull * n_scenes);
     SPH_ALLOC(h->valBits, 1);
     SPH_ALLOC(h->valIdx, 8ull * n_scenes);
-    if (!(flags & SPH_FLAG_CELL_WALK)) SPH_ALLOC(h->mask, kMaskWords * std::max<uint64_t>(N, 1));
+    if (!(flags & SPH_FLAG_CELL_WALK)) SPH_ALLOC(h->mask, kMaskWords * std::max<uint64_t>(N, 1));  // tile and run walks
     if (flags & SPH_FLAG_RECORD_DEBUG) {
         SPH_ALLOC(h->dens, std::max<uint64_t>(N, 1));
         SPH_ALLOC(h->acc, std::max<uint64_t>(N, 1));

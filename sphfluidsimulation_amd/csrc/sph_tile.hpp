@@ -1,0 +1,427 @@
+// LDS-staged neighbour passes ("tile walk", the default).
+//
+// A workgroup owns 256 consecutive particles in bucket order. Before any
+// neighbour is read it builds, in LDS, the set of distinct buckets its
+// particles' 27 neighbour cells map to (an open-addressing map keyed by bucket),
+// then copies those buckets' particles from HBM into LDS once. Each particle
+// then walks its 27 neighbour cells in the reference's order (SF:397-404),
+// looks each cell's bucket up in the map and reads that bucket's candidates
+// from LDS. A bucket that two of the 27 cells collide on is walked twice, as
+// the reference does (P5 of the survey); a bucket that does not fit in LDS, or
+// did not fit in the map, is read from HBM instead, so every M works.
+//
+// Particles in the same cell share their 27 buckets, so only the first
+// particle of each cell inserts them (a 256-particle tile holds 32-64 cells
+// and needs about 144 distinct buckets on a fluid lattice at spacing h/2).
+//
+// The density pass records which of a particle's first 256 candidates (in walk
+// order) lie within h, with a widened test; the force pass walks the same
+// buckets in the same order and only evaluates the recorded candidates (plus
+// every candidate past the 256th). SF = SPHFluidSimulation/source/SPH/Kernels/
+// SPHFunctions.cpp.
+#pragma once
+
+#include "sph_kernels.hpp"
+
+namespace sph {
+
+constexpr uint32_t kMapSlots = 512;      // bucket map slots per workgroup
+constexpr uint32_t kMapProbe = 32;       // linear-probe limit; past it a bucket stays unmapped
+constexpr uint32_t kCapDensity = 1536;   // staged candidates of the density pass (16 B each)
+constexpr uint32_t kCapForce = 1024;     // staged candidates of the force pass (32 B each)
+constexpr uint32_t kNotStaged = 0xFFFFFFFFu;
+static_assert(kMapSlots == 2 * kBlock, "staging assigns two map slots per thread");
+
+__device__ inline uint32_t map_hash(uint32_t b) { return (b * 2654435761u) >> 23; }  // 9 bits
+
+// cs.x[a] | cs.y[b] | cs.z[c] with selects instead of indexed loads, so the
+// rolled 27-cell loops keep the spread coordinates in registers.
+__device__ inline uint32_t pick3(const uint32_t (&v)[3], int k) { return k == 0 ? v[0] : (k == 1 ? v[1] : v[2]); }
+__device__ inline uint32_t cell_code(const CellSpread& cs, int a, int b, int c) {
+    return pick3(cs.x, a) | pick3(cs.y, b) | pick3(cs.z, c);
+}
+
+// Bucket map in LDS: key = bucket + 1 (0 = empty), then the bucket's global
+// particle range and its LDS offset (kNotStaged if it lives in HBM only).
+struct BucketMap {
+    uint32_t key[kMapSlots];
+    uint32_t beg[kMapSlots];
+    uint32_t cnt[kMapSlots];
+    uint32_t off[kMapSlots];
+};
+
+__device__ inline void map_insert(BucketMap& m, uint32_t b) {
+    uint32_t s = map_hash(b);
+    for (uint32_t p = 0; p < kMapProbe; ++p) {
+        const uint32_t prev = atomicCAS(&m.key[s], 0u, b + 1u);
+        if (prev == 0u || prev == b + 1u) return;
+        s = (s + 1u) & (kMapSlots - 1u);
+    }
+}
+
+__device__ inline int map_find(const BucketMap& m, uint32_t b) {
+    uint32_t s = map_hash(b);
+    for (uint32_t p = 0; p < kMapProbe; ++p) {
+        const uint32_t k = m.key[s];
+        if (k == b + 1u) return (int)s;
+        if (k == 0u) return -1;
+        s = (s + 1u) & (kMapSlots - 1u);
+    }
+    return -1;
+}
+
+// One neighbour bucket of the walk: global range [beg, beg + cnt) and the LDS
+// offset of its staged copy.
+struct BucketView {
+    uint32_t beg, cnt, off;
+};
+
+__device__ inline BucketView bucket_view(const BucketMap& m, uint32_t bucket, const uint32_t* __restrict__ table,
+                                         uint32_t lo, uint32_t hi) {
+    BucketView v;
+    const int slot = map_find(m, bucket);
+    if (slot >= 0) {
+        v.beg = m.beg[slot];
+        v.cnt = m.cnt[slot];
+        v.off = m.off[slot];
+    } else {
+        const uint32_t tb = table[bucket], te = table[bucket + 1];
+        const uint32_t s = tb > lo ? tb : lo, e = te < hi ? te : hi;
+        v.beg = s;
+        v.cnt = e > s ? e - s : 0u;
+        v.off = kNotStaged;
+    }
+    return v;
+}
+
+// Builds the bucket map of a tile and stages the mapped buckets' particles.
+// `valid` lanes hold particle i in cell code `code`; every lane must call this
+// (it contains barriers).
+template <bool kVel>
+__device__ inline void stage_tile(BucketMap& m, uint32_t* s_code, uint32_t* s_w, bool valid, const CellSpread& cs,
+                                  uint32_t code, const SceneDev& S, const uint32_t* __restrict__ table,
+                                  const float4* __restrict__ pos, const float4* __restrict__ vel, float4* candP,
+                                  float4* candV, uint32_t cap) {
+    const uint32_t tid = threadIdx.x;
+    m.key[tid] = 0u;
+    m.key[tid + kBlock] = 0u;
+    s_code[tid] = valid ? code : 0xFFFFFFFFu;
+    __syncthreads();
+    const uint32_t lo = S.pBegin, hi = S.pBegin + S.pCount;
+    // The first particle of each cell inserts the cell's 27 buckets.
+    if (valid && (tid == 0 || s_code[tid - 1] != code)) {
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                for (int c = 0; c < 3; ++c) map_insert(m, fastmod(cell_code(cs, a, b, c), S.magicM, S.M));
+    }
+    __syncthreads();
+    // Two slots per thread: range, count, then an exclusive scan for LDS offsets.
+    uint32_t cnt2[2], beg2[2], sum = 0;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const uint32_t slot = 2u * tid + (uint32_t)q;
+        const uint32_t k = m.key[slot];
+        cnt2[q] = 0u;
+        beg2[q] = 0u;
+        if (k) {
+            const uint32_t bk = k - 1u;
+            const uint32_t tb = table[bk], te = table[bk + 1];
+            const uint32_t s = tb > lo ? tb : lo, e = te < hi ? te : hi;
+            beg2[q] = s;
+            cnt2[q] = e > s ? e - s : 0u;
+        }
+        sum += cnt2[q];
+    }
+    uint32_t total;
+    uint32_t o = block_exclusive_scan(sum, total, s_w);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const uint32_t slot = 2u * tid + (uint32_t)q;
+        const bool fits = o + cnt2[q] <= cap;
+        m.beg[slot] = beg2[q];
+        m.cnt[slot] = cnt2[q];
+        m.off[slot] = fits ? o : kNotStaged;
+        if (fits)
+            for (uint32_t t = 0; t < cnt2[q]; ++t) {
+                candP[o + t] = pos[beg2[q] + t];
+                if (kVel) candV[o + t] = vel[beg2[q] + t];
+            }
+        o += cnt2[q];
+    }
+    __syncthreads();
+}
+
+template <bool kStatic, bool kDebug>
+__global__ __launch_bounds__(kBlock) void k_density_tile(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
+                                                         const SceneDev* __restrict__ scenes, float4* pos,
+                                                         const uint32_t* __restrict__ tab,
+                                                         const float4* __restrict__ spos,
+                                                         const uint32_t* __restrict__ stab,
+                                                         uint32_t* __restrict__ mask, float* __restrict__ dens) {
+    __shared__ BucketMap m;
+    __shared__ float4 candP[kCapDensity];
+    __shared__ uint32_t s_code[kBlock];
+    __shared__ uint32_t s_w[4];
+    const uint32_t tile = xcd_tile(blockIdx.x, numBlocks);
+    if (tile >= numBlocks) return;  // whole workgroup leaves together
+    const BlockDesc bd = blocks[tile];
+    const SceneDev& S = scenes[bd.scene];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t i = bd.begin + tid;
+    const bool valid = i < bd.end;
+    const float4 xi = valid ? pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float h = S.h, h2 = S.h2;
+    const float h2wide = h2 * 1.0001f;
+    const uint32_t cx = cell_coord(xi.x, h), cy = cell_coord(xi.y, h), cz = cell_coord(xi.z, h);
+    const CellSpread cs = cell_spread(xi.x, xi.y, xi.z, h);
+    const uint32_t code = morton3(cx, cy, cz);
+    const uint32_t* table = tab + S.tBegin;
+    const uint32_t lo = S.pBegin, hi = S.pBegin + S.pCount;
+    stage_tile<false>(m, s_code, s_w, valid, cs, code, S, table, pos, nullptr, candP, nullptr, kCapDensity);
+    if (!valid) return;
+
+    float sumDyn = 0.f, sumSt = 0.f;
+    uint32_t k = 0, bits = 0;
+    uint32_t* mrow = mask + (size_t)i * kMaskWords;
+    auto candidate = [&](uint32_t j, const float4& xj) {
+        const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const bool self = j == i;
+        if (r2 <= h2wide && !self) bits |= 1u << (k & 31u);
+        if ((k & 31u) == 31u) {
+            if (k < kMaskBits) mrow[k >> 5] = bits;
+            bits = 0u;
+        }
+        ++k;
+        if (!self && r2 <= h2) {
+            const float r = __builtin_amdgcn_sqrtf(r2);
+            const float q = h - r;
+            sumDyn += (r < h) ? q * q * q : 0.f;
+        }
+    };
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            for (int c = 0; c < 3; ++c) {
+                const uint32_t bucket = fastmod(cell_code(cs, a, b, c), S.magicM, S.M);
+                const BucketView v = bucket_view(m, bucket, table, lo, hi);
+                if (v.off != kNotStaged) {
+                    for (uint32_t t = 0; t < v.cnt; t += 4u) {
+                        const uint32_t last = v.cnt - 1u;
+                        float4 xs[4];
+#pragma unroll
+                        for (uint32_t q = 0; q < 4; ++q) xs[q] = candP[v.off + min(t + q, last)];
+#pragma unroll
+                        for (uint32_t q = 0; q < 4; ++q)
+                            if (t + q < v.cnt) candidate(v.beg + t + q, xs[q]);
+                    }
+                } else {
+                    for (uint32_t t = 0; t < v.cnt; ++t) candidate(v.beg + t, pos[v.beg + t]);
+                }
+            }
+    // Flush the open mask word and clear the words this particle never reached.
+    if ((k & 31u) != 0u && k < kMaskBits) mrow[k >> 5] = bits;
+    for (uint32_t w = (k + 31u) >> 5; w < kMaskWords; ++w) mrow[w] = 0u;
+    if (kStatic && S.sCount) {
+        walk_runs(cx, cy, cz, S.Ms, S.magicMs, stab + S.stBegin, S.sBegin, S.sBegin + S.sCount,
+                  [&](uint32_t s, uint32_t e) {
+                      for (uint32_t j = s; j < e; ++j) {
+                          const float4 xj = spos[j];
+                          const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+                          const float r2 = dx * dx + dy * dy + dz * dz;
+                          if (r2 > h2) continue;
+                          const float r = __builtin_amdgcn_sqrtf(r2);
+                          const float q = h - r;
+                          sumSt += (r < h) ? q * q * q : 0.f;
+                      }
+                  });
+    }
+    const float rho = S.selfDensity + (sumDyn * S.mass + sumSt * S.mass) * S.C;
+    // Only this lane writes pos[i].w; no lane of this pass reads any .w.
+    reinterpret_cast<float*>(pos)[4ull * i + 3] = S.k * (rho - S.rho0);
+    if (kDebug) dens[i] = rho;
+}
+
+template <bool kStatic, bool kDebug, bool kTri>
+__global__ __launch_bounds__(kBlock) void k_force_tile(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
+                                                       const SceneDev* __restrict__ scenes,
+                                                       const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                                       const uint32_t* __restrict__ tab,
+                                                       const float4* __restrict__ spos,
+                                                       const uint32_t* __restrict__ stab,
+                                                       const uint32_t* __restrict__ mask,
+                                                       const uint32_t* __restrict__ ids, const float* __restrict__ tris,
+                                                       float dt, float4* __restrict__ posT, float4* __restrict__ velT,
+                                                       uint32_t* __restrict__ counts, float4* __restrict__ accOut) {
+    __shared__ BucketMap m;
+    __shared__ float4 candP[kCapForce];
+    __shared__ float4 candV[kCapForce];
+    __shared__ uint32_t smask[(kMaskWords + 1) * kBlock];
+    __shared__ uint32_t s_code[kBlock];
+    __shared__ uint32_t s_w[4];
+    const uint32_t tile = xcd_tile(blockIdx.x, numBlocks);
+    if (tile >= numBlocks) return;
+    const BlockDesc bd = blocks[tile];
+    const SceneDev& S = scenes[bd.scene];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t i = bd.begin + tid;
+    const bool valid = i < bd.end;
+    if (valid) {
+        const uint4* mr = reinterpret_cast<const uint4*>(mask + (size_t)i * kMaskWords);
+        const uint4 m0 = mr[0], m1 = mr[1];
+        smask[0 * kBlock + tid] = m0.x;
+        smask[1 * kBlock + tid] = m0.y;
+        smask[2 * kBlock + tid] = m0.z;
+        smask[3 * kBlock + tid] = m0.w;
+        smask[4 * kBlock + tid] = m1.x;
+        smask[5 * kBlock + tid] = m1.y;
+        smask[6 * kBlock + tid] = m1.z;
+        smask[7 * kBlock + tid] = m1.w;
+        smask[8 * kBlock + tid] = 0u;
+    }
+    const float4 xi = valid ? pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 vi = valid ? vel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float pi = xi.w;  // this step's pressure (density pass)
+    const float h = S.h, h2 = S.h2;
+    const uint32_t cx = cell_coord(xi.x, h), cy = cell_coord(xi.y, h), cz = cell_coord(xi.z, h);
+    const CellSpread cs = cell_spread(xi.x, xi.y, xi.z, h);
+    const uint32_t code = morton3(cx, cy, cz);
+    const uint32_t* table = tab + S.tBegin;
+    const uint32_t lo = S.pBegin, hi = S.pBegin + S.pCount;
+    stage_tile<true>(m, s_code, s_w, valid, cs, code, S, table, pos, vel, candP, candV, kCapForce);
+    if (!valid) return;
+
+    const float rho = pi / S.k + S.rho0;  // SF:544
+    const float absP = fabsf(pi);
+    float fpx = 0.f, fpy = 0.f, fpz = 0.f;
+    float fvx = 0.f, fvy = 0.f, fvz = 0.f;
+    float fsx = 0.f, fsy = 0.f, fsz = 0.f;
+    float sumD2s = 0.f;
+    // One dynamic neighbour j (SF:557-589).
+    auto pair = [&](uint32_t j, const float4& xj, const float4& vj) {
+        float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (j == i || r2 > h2) return;
+        const float r = __builtin_amdgcn_sqrtf(r2);
+        if (r2 == 0.f || r == 0.f) {
+            const float3 d = random_direction((float)(ids[i] - S.pBegin));
+            dx = d.x;
+            dy = d.y;
+            dz = d.z;
+        } else {
+            const float inv = __builtin_amdgcn_rcpf(r);
+            dx *= inv;
+            dy *= inv;
+            dz *= inv;
+        }
+        const float q = h - r;
+        const bool in = r < h;
+        const float w1 = in ? -3.0f * q * q : 0.f;
+        const float w2 = in ? 6.0f * q : 0.f;
+        const float sp = (pi + xj.w) * w1;
+        fpx += dx * sp;
+        fpy += dy * sp;
+        fpz += dz * sp;
+        fvx += (vj.x - vi.x) * w2;
+        fvy += (vj.y - vi.y) * w2;
+        fvz += (vj.z - vi.z) * w2;
+    };
+    uint32_t k0 = 0;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            for (int c = 0; c < 3; ++c) {
+                const uint32_t bucket = fastmod(cell_code(cs, a, b, c), S.magicM, S.M);
+                const BucketView v = bucket_view(m, bucket, table, lo, hi);
+                for (uint32_t off = 0; off < v.cnt; off += 32u) {
+                    const uint32_t kk = k0 + off;
+                    uint32_t bits;
+                    if (kk + 32u <= kMaskBits) {
+                        const uint32_t w = kk >> 5;
+                        bits = __builtin_amdgcn_alignbit(smask[(w + 1) * kBlock + tid], smask[w * kBlock + tid],
+                                                         kk & 31u);
+                    } else if (kk >= kMaskBits) {
+                        bits = ~0u;  // past the recorded candidates: test them all
+                    } else {
+                        const uint32_t w = kk >> 5;
+                        const uint32_t rec = kMaskBits - kk;
+                        const uint32_t mm = __builtin_amdgcn_alignbit(smask[(w + 1) * kBlock + tid],
+                                                                      smask[w * kBlock + tid], kk & 31u);
+                        bits = (mm & ((1u << rec) - 1u)) | ~((1u << rec) - 1u);
+                    }
+                    if (v.cnt - off < 32u) bits &= (1u << (v.cnt - off)) - 1u;
+                    if (v.off != kNotStaged) {
+                        while (bits) {
+                            const uint32_t t = off + (uint32_t)__builtin_ctz(bits);
+                            bits &= bits - 1u;
+                            pair(v.beg + t, candP[v.off + t], candV[v.off + t]);
+                        }
+                    } else {
+                        while (bits) {
+                            const uint32_t t = off + (uint32_t)__builtin_ctz(bits);
+                            bits &= bits - 1u;
+                            pair(v.beg + t, pos[v.beg + t], vel[v.beg + t]);
+                        }
+                    }
+                }
+                k0 += v.cnt;
+            }
+    if (kStatic && S.sCount) {
+        walk_runs(cx, cy, cz, S.Ms, S.magicMs, stab + S.stBegin, S.sBegin, S.sBegin + S.sCount,
+                  [&](uint32_t s, uint32_t e) {
+                      for (uint32_t j = s; j < e; ++j) {
+                          const float4 xj = spos[j];
+                          float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+                          const float r2 = dx * dx + dy * dy + dz * dz;
+                          if (r2 > h2) continue;
+                          const float r = __builtin_amdgcn_sqrtf(r2);
+                          if (r2 == 0.f || r == 0.f) {
+                              const float3 d = random_direction((float)(ids[i] - S.pBegin));
+                              dx = d.x;
+                              dy = d.y;
+                              dz = d.z;
+                          } else {
+                              const float inv = __builtin_amdgcn_rcpf(r);
+                              dx *= inv;
+                              dy *= inv;
+                              dz *= inv;
+                          }
+                          const float q = h - r;
+                          const bool in = r < h;
+                          const float sp = absP * (in ? -3.0f * q * q : 0.f);
+                          fsx += dx * sp;
+                          fsy += dy * sp;
+                          fsz += dz * sp;
+                          sumD2s += in ? 6.0f * q : 0.f;
+                      }
+                  });
+    }
+    // SF:626-638
+    const float sP = S.mass / (2.0f * rho) * S.C;
+    const float sV = S.mu * S.mass * S.C;
+    const float sVs = S.mu * S.mass * S.C * 0.0f;
+    float Fx = fpx * sP + fvx * sV + fsx * sP + (-vi.x * sumD2s) * sVs;
+    float Fy = fpy * sP + fvy * sV + fsy * sP + (-vi.y * sumD2s) * sVs;
+    float Fz = fpz * sP + fvz * sV + fsz * sP + (-vi.z * sumD2s) * sVs;
+    if (kTri) {
+        const float3 p3 = make_float3(xi.x, xi.y, xi.z);
+        for (uint32_t t = 0; t < S.triCount; ++t) {
+            const float3 f = triangle_force(p3, tris + 9u * (S.triBegin + t));
+            Fx += f.x;
+            Fy += f.y;
+            Fz += f.z;
+        }
+    }
+    const float ax = Fx / rho + S.gx, ay = Fy / rho + S.gy, az = Fz / rho + S.gz;
+    if (kDebug) accOut[i] = make_float4(ax, ay, az, 0.f);
+    // SF:641-645: symplectic Euler, then the key of the new cell.
+    const float nvx = vi.x + ax * dt, nvy = vi.y + ay * dt, nvz = vi.z + az * dt;
+    const float nx = xi.x + nvx * dt, ny = xi.y + nvy * dt, nz = xi.z + nvz * dt;
+    const uint32_t ncode = morton3(cell_coord(nx, h), cell_coord(ny, h), cell_coord(nz, h));
+    const uint32_t key = fastmod(ncode, S.magicM, S.M);
+    posT[i] = make_float4(nx, ny, nz, pi);
+    velT[i] = make_float4(nvx, nvy, nvz, __uint_as_float(key));
+    const uint32_t gk = S.tBegin + key;
+    const Run run = wave_run(gk);
+    if (run.is_head) atomicAdd(&counts[gk], run.len);
+}
+
+}  // namespace sph

@@ -19,7 +19,8 @@ import pytest
 import oracle as orc
 from helpers import bucket_multisets_equal, check_buckets, expected_starts, max_norm_rel, max_rel
 from sphfluidsimulation_amd import SPHError, SimulationEngineHIP, scenes
-from sphfluidsimulation_amd._lib import SPH_E_STATE, SPH_FLAG_CELL_WALK, SPH_FLAG_HIP_GRAPH, SPH_FLAG_RECORD_DEBUG
+from sphfluidsimulation_amd._lib import (SPH_E_STATE, SPH_FLAG_CELL_WALK, SPH_FLAG_HIP_GRAPH, SPH_FLAG_RECORD_DEBUG,
+                                         SPH_FLAG_RUN_WALK)
 
 pytestmark = pytest.mark.gpu
 
@@ -27,9 +28,10 @@ DENS_TOL = 1e-5
 ACC_TOL = 1e-5
 FP64_DENS_TOL = 1e-4
 
-# The two neighbour walks: Morton runs with the candidate mask (default) and the
-# reference's cell-by-cell loop.
-WALKS = [pytest.param(0, id="runs"), pytest.param(SPH_FLAG_CELL_WALK, id="cells")]
+# The three neighbour-pass variants: LDS-staged bucket tiles (default), Morton
+# runs from HBM, and the reference's cell-by-cell loop from HBM.
+WALKS = [pytest.param(0, id="tile"), pytest.param(SPH_FLAG_RUN_WALK, id="runs"),
+         pytest.param(SPH_FLAG_CELL_WALK, id="cells")]
 
 
 def say(msg):
