@@ -567,6 +567,7 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
     SPH_ALLOC(h->consOut, 8ull * n_scenes);
     SPH_ALLOC(h->valBits, 1);
     SPH_ALLOC(h->valIdx, 8ull * n_scenes);
+    static_assert(kMaskWords == kTileMaskWords, "both mask formats use 8 words per particle");
     if (!(flags & SPH_FLAG_CELL_WALK)) SPH_ALLOC(h->mask, kMaskWords * std::max<uint64_t>(N, 1));  // tile and run walks
     if (flags & SPH_FLAG_RECORD_DEBUG) {
         SPH_ALLOC(h->dens, std::max<uint64_t>(N, 1));

@@ -14,11 +14,12 @@
 // particle of each cell inserts them (a 256-particle tile holds 32-64 cells
 // and needs about 144 distinct buckets on a fluid lattice at spacing h/2).
 //
-// The density pass records which of a particle's first 256 candidates (in walk
-// order) lie within h, with a widened test; the force pass walks the same
-// buckets in the same order and only evaluates the recorded candidates (plus
-// every candidate past the 256th). SF = SPHFluidSimulation/source/SPH/Kernels/
-// SPHFunctions.cpp.
+// The density pass first tests each bucket's candidates against a widened
+// radius only, records the in-range ones in a per-particle byte mask, and then
+// evaluates the kernel for the flagged candidates alone, so the square root and
+// cube are paid per neighbour rather than per candidate. The force pass reads
+// the mask and evaluates only the recorded neighbours. SF =
+// SPHFluidSimulation/source/SPH/Kernels/SPHFunctions.cpp.
 #pragma once
 
 #include "sph_kernels.hpp"
@@ -27,7 +28,7 @@ namespace sph {
 
 constexpr uint32_t kMapSlots = 512;      // bucket map slots per workgroup
 constexpr uint32_t kMapProbe = 32;       // linear-probe limit; past it a bucket stays unmapped
-constexpr uint32_t kCapDensity = 1536;   // staged candidates of the density pass (16 B each)
+constexpr uint32_t kCapDensity = 1280;   // staged candidates of the density pass (16 B each)
 constexpr uint32_t kCapForce = 1024;     // staged candidates of the force pass (32 B each)
 constexpr uint32_t kNotStaged = 0xFFFFFFFFu;
 static_assert(kMapSlots == 2 * kBlock, "staging assigns two map slots per thread");
@@ -142,13 +143,36 @@ __device__ inline void stage_tile(BucketMap& m, uint32_t* s_code, uint32_t* s_w,
         m.cnt[slot] = cnt2[q];
         m.off[slot] = fits ? o : kNotStaged;
         if (fits)
-            for (uint32_t t = 0; t < cnt2[q]; ++t) {
-                candP[o + t] = pos[beg2[q] + t];
-                if (kVel) candV[o + t] = vel[beg2[q] + t];
+            for (uint32_t t = 0; t < cnt2[q]; t += 4u) {
+                float4 p4[4], v4[4];
+                const uint32_t last = cnt2[q] - 1u;
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u) {
+                    p4[u] = pos[beg2[q] + min(t + u, last)];
+                    if (kVel) v4[u] = vel[beg2[q] + min(t + u, last)];
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u)
+                    if (t + u < cnt2[q]) {
+                        candP[o + t + u] = p4[u];
+                        if (kVel) candV[o + t + u] = v4[u];
+                    }
             }
         o += cnt2[q];
     }
     __syncthreads();
+}
+
+// Candidate mask of the tile walk: byte v (v = 0..26, the neighbour cells in
+// the reference's x-y-z loop order) flags which of the first 8 candidates of
+// that cell's bucket lie within h (widened test). Candidates past the 8th of a
+// bucket are not recorded; the force pass tests them itself. 27 bytes padded
+// to 8 words per particle.
+constexpr uint32_t kTileMaskWords = 8;
+
+// Staged-or-HBM read of candidate t of a bucket view.
+__device__ inline float4 cand_pos(const BucketView& v, uint32_t t, const float4* candP, const float4* pos) {
+    return v.off != kNotStaged ? candP[v.off + t] : pos[v.beg + t];
 }
 
 template <bool kStatic, bool kDebug>
@@ -160,6 +184,7 @@ __global__ __launch_bounds__(kBlock) void k_density_tile(const BlockDesc* __rest
                                                          uint32_t* __restrict__ mask, float* __restrict__ dens) {
     __shared__ BucketMap m;
     __shared__ float4 candP[kCapDensity];
+    __shared__ uint32_t smask[kTileMaskWords * kBlock];
     __shared__ uint32_t s_code[kBlock];
     __shared__ uint32_t s_w[4];
     const uint32_t tile = xcd_tile(blockIdx.x, numBlocks);
@@ -181,46 +206,50 @@ __global__ __launch_bounds__(kBlock) void k_density_tile(const BlockDesc* __rest
     if (!valid) return;
 
     float sumDyn = 0.f, sumSt = 0.f;
-    uint32_t k = 0, bits = 0;
-    uint32_t* mrow = mask + (size_t)i * kMaskWords;
-    auto candidate = [&](uint32_t j, const float4& xj) {
+    // Exact in-range contribution of candidate t of bucket v (SF:426-453).
+    auto d0 = [&](const BucketView& v, uint32_t t) {
+        const float4 xj = cand_pos(v, t, candP, pos);
         const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
         const float r2 = dx * dx + dy * dy + dz * dz;
-        const bool self = j == i;
-        if (r2 <= h2wide && !self) bits |= 1u << (k & 31u);
-        if ((k & 31u) == 31u) {
-            if (k < kMaskBits) mrow[k >> 5] = bits;
-            bits = 0u;
-        }
-        ++k;
-        if (!self && r2 <= h2) {
+        if (r2 <= h2 && v.beg + t != i) {
             const float r = __builtin_amdgcn_sqrtf(r2);
             const float q = h - r;
             sumDyn += (r < h) ? q * q * q : 0.f;
         }
     };
+    uint32_t word = 0;
     for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b)
             for (int c = 0; c < 3; ++c) {
+                const uint32_t visit = (uint32_t)(a * 9 + b * 3 + c);
                 const uint32_t bucket = fastmod(cell_code(cs, a, b, c), S.magicM, S.M);
                 const BucketView v = bucket_view(m, bucket, table, lo, hi);
-                if (v.off != kNotStaged) {
-                    for (uint32_t t = 0; t < v.cnt; t += 4u) {
-                        const uint32_t last = v.cnt - 1u;
-                        float4 xs[4];
+                const uint32_t n8 = v.cnt < 8u ? v.cnt : 8u;
+                const uint32_t selfT = i - v.beg;  // >= cnt unless i is in this bucket
+                // First 8 candidates: widened test only, four loads in flight.
+                uint32_t bits8 = 0;
+                for (uint32_t t = 0; t < n8; t += 4u) {
+                    float4 xs[4];
 #pragma unroll
-                        for (uint32_t q = 0; q < 4; ++q) xs[q] = candP[v.off + min(t + q, last)];
+                    for (uint32_t q = 0; q < 4; ++q) xs[q] = cand_pos(v, min(t + q, n8 - 1u), candP, pos);
 #pragma unroll
-                        for (uint32_t q = 0; q < 4; ++q)
-                            if (t + q < v.cnt) candidate(v.beg + t + q, xs[q]);
+                    for (uint32_t q = 0; q < 4; ++q) {
+                        const float dx = xs[q].x - xi.x, dy = xs[q].y - xi.y, dz = xs[q].z - xi.z;
+                        const float r2 = dx * dx + dy * dy + dz * dz;
+                        const bool in = (t + q < n8) && (t + q != selfT) && r2 <= h2wide;
+                        bits8 |= in ? (1u << (t + q)) : 0u;
                     }
-                } else {
-                    for (uint32_t t = 0; t < v.cnt; ++t) candidate(v.beg + t, pos[v.beg + t]);
+                }
+                // Then the exact contribution of the flagged ones.
+                for (uint32_t mb = bits8; mb; mb &= mb - 1u) d0(v, (uint32_t)__builtin_ctz(mb));
+                // Candidates past the 8th (crowded buckets) are handled directly.
+                for (uint32_t t = 8; t < v.cnt; ++t) d0(v, t);
+                word |= bits8 << (8u * (visit & 3u));
+                if ((visit & 3u) == 3u || visit == 26u) {
+                    smask[(visit >> 2) * kBlock + tid] = word;
+                    word = 0u;
                 }
             }
-    // Flush the open mask word and clear the words this particle never reached.
-    if ((k & 31u) != 0u && k < kMaskBits) mrow[k >> 5] = bits;
-    for (uint32_t w = (k + 31u) >> 5; w < kMaskWords; ++w) mrow[w] = 0u;
     if (kStatic && S.sCount) {
         walk_runs(cx, cy, cz, S.Ms, S.magicMs, stab + S.stBegin, S.sBegin, S.sBegin + S.sCount,
                   [&](uint32_t s, uint32_t e) {
@@ -239,6 +268,10 @@ __global__ __launch_bounds__(kBlock) void k_density_tile(const BlockDesc* __rest
     // Only this lane writes pos[i].w; no lane of this pass reads any .w.
     reinterpret_cast<float*>(pos)[4ull * i + 3] = S.k * (rho - S.rho0);
     if (kDebug) dens[i] = rho;
+    uint4* mrow = reinterpret_cast<uint4*>(mask + (size_t)i * kTileMaskWords);
+    mrow[0] = make_uint4(smask[0 * kBlock + tid], smask[1 * kBlock + tid], smask[2 * kBlock + tid],
+                         smask[3 * kBlock + tid]);
+    mrow[1] = make_uint4(smask[4 * kBlock + tid], smask[5 * kBlock + tid], smask[6 * kBlock + tid], 0u);
 }
 
 template <bool kStatic, bool kDebug, bool kTri>
@@ -255,7 +288,7 @@ __global__ __launch_bounds__(kBlock) void k_force_tile(const BlockDesc* __restri
     __shared__ BucketMap m;
     __shared__ float4 candP[kCapForce];
     __shared__ float4 candV[kCapForce];
-    __shared__ uint32_t smask[(kMaskWords + 1) * kBlock];
+    __shared__ uint32_t smask[kTileMaskWords * kBlock];
     __shared__ uint32_t s_code[kBlock];
     __shared__ uint32_t s_w[4];
     const uint32_t tile = xcd_tile(blockIdx.x, numBlocks);
@@ -266,7 +299,7 @@ __global__ __launch_bounds__(kBlock) void k_force_tile(const BlockDesc* __restri
     const uint32_t i = bd.begin + tid;
     const bool valid = i < bd.end;
     if (valid) {
-        const uint4* mr = reinterpret_cast<const uint4*>(mask + (size_t)i * kMaskWords);
+        const uint4* mr = reinterpret_cast<const uint4*>(mask + (size_t)i * kTileMaskWords);
         const uint4 m0 = mr[0], m1 = mr[1];
         smask[0 * kBlock + tid] = m0.x;
         smask[1 * kBlock + tid] = m0.y;
@@ -275,8 +308,6 @@ __global__ __launch_bounds__(kBlock) void k_force_tile(const BlockDesc* __restri
         smask[4 * kBlock + tid] = m1.x;
         smask[5 * kBlock + tid] = m1.y;
         smask[6 * kBlock + tid] = m1.z;
-        smask[7 * kBlock + tid] = m1.w;
-        smask[8 * kBlock + tid] = 0u;
     }
     const float4 xi = valid ? pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 vi = valid ? vel[i] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -296,11 +327,14 @@ __global__ __launch_bounds__(kBlock) void k_force_tile(const BlockDesc* __restri
     float fvx = 0.f, fvy = 0.f, fvz = 0.f;
     float fsx = 0.f, fsy = 0.f, fsz = 0.f;
     float sumD2s = 0.f;
-    // One dynamic neighbour j (SF:557-589).
-    auto pair = [&](uint32_t j, const float4& xj, const float4& vj) {
+    // One dynamic candidate t of bucket v (SF:557-589).
+    auto pair = [&](const BucketView& v, uint32_t t) {
+        const bool staged = v.off != kNotStaged;
+        const float4 xj = staged ? candP[v.off + t] : pos[v.beg + t];
         float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
         const float r2 = dx * dx + dy * dy + dz * dz;
-        if (j == i || r2 > h2) return;
+        if (v.beg + t == i || r2 > h2) return;
+        const float4 vj = staged ? candV[v.off + t] : vel[v.beg + t];
         const float r = __builtin_amdgcn_sqrtf(r2);
         if (r2 == 0.f || r == 0.f) {
             const float3 d = random_direction((float)(ids[i] - S.pBegin));
@@ -325,44 +359,15 @@ __global__ __launch_bounds__(kBlock) void k_force_tile(const BlockDesc* __restri
         fvy += (vj.y - vi.y) * w2;
         fvz += (vj.z - vi.z) * w2;
     };
-    uint32_t k0 = 0;
     for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b)
             for (int c = 0; c < 3; ++c) {
+                const uint32_t visit = (uint32_t)(a * 9 + b * 3 + c);
                 const uint32_t bucket = fastmod(cell_code(cs, a, b, c), S.magicM, S.M);
                 const BucketView v = bucket_view(m, bucket, table, lo, hi);
-                for (uint32_t off = 0; off < v.cnt; off += 32u) {
-                    const uint32_t kk = k0 + off;
-                    uint32_t bits;
-                    if (kk + 32u <= kMaskBits) {
-                        const uint32_t w = kk >> 5;
-                        bits = __builtin_amdgcn_alignbit(smask[(w + 1) * kBlock + tid], smask[w * kBlock + tid],
-                                                         kk & 31u);
-                    } else if (kk >= kMaskBits) {
-                        bits = ~0u;  // past the recorded candidates: test them all
-                    } else {
-                        const uint32_t w = kk >> 5;
-                        const uint32_t rec = kMaskBits - kk;
-                        const uint32_t mm = __builtin_amdgcn_alignbit(smask[(w + 1) * kBlock + tid],
-                                                                      smask[w * kBlock + tid], kk & 31u);
-                        bits = (mm & ((1u << rec) - 1u)) | ~((1u << rec) - 1u);
-                    }
-                    if (v.cnt - off < 32u) bits &= (1u << (v.cnt - off)) - 1u;
-                    if (v.off != kNotStaged) {
-                        while (bits) {
-                            const uint32_t t = off + (uint32_t)__builtin_ctz(bits);
-                            bits &= bits - 1u;
-                            pair(v.beg + t, candP[v.off + t], candV[v.off + t]);
-                        }
-                    } else {
-                        while (bits) {
-                            const uint32_t t = off + (uint32_t)__builtin_ctz(bits);
-                            bits &= bits - 1u;
-                            pair(v.beg + t, pos[v.beg + t], vel[v.beg + t]);
-                        }
-                    }
-                }
-                k0 += v.cnt;
+                const uint32_t bits8 = (smask[(visit >> 2) * kBlock + tid] >> (8u * (visit & 3u))) & 0xFFu;
+                for (uint32_t mb = bits8; mb; mb &= mb - 1u) pair(v, (uint32_t)__builtin_ctz(mb));
+                for (uint32_t t = 8; t < v.cnt; ++t) pair(v, t);
             }
     if (kStatic && S.sCount) {
         walk_runs(cx, cy, cz, S.Ms, S.magicMs, stab + S.stBegin, S.sBegin, S.sBegin + S.sCount,
