@@ -107,8 +107,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="1m", choices=sorted(CONFIGS))
     ap.add_argument("--graph", action="store_true", help="replay steps from hipGraphs")
-    ap.add_argument("--walk", default="tile", choices=["tile", "runs", "cells"],
-                    help="neighbour passes: LDS-staged bucket tiles (default), HBM Morton runs, or cell by cell")
+    ap.add_argument("--walk", default="cellwave", choices=["cellwave", "tile", "runs", "cells"],
+                    help="neighbour passes: cell-group waves (default), LDS-staged bucket tiles, "
+                         "HBM Morton runs, or cell by cell")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--profile-all", action="store_true", help="time every stage with events (default: dominant)")
@@ -123,7 +124,8 @@ def main():
 
 
     from sphfluidsimulation_amd import SimulationEngineHIP, scenes
-    from sphfluidsimulation_amd._lib import SPH_FLAG_CELL_WALK, SPH_FLAG_HIP_GRAPH, SPH_FLAG_RUN_WALK
+    from sphfluidsimulation_amd._lib import (SPH_FLAG_CELL_WALK, SPH_FLAG_HIP_GRAPH, SPH_FLAG_RUN_WALK,
+                                             SPH_FLAG_TILE_WALK)
 
     if args.copy_probe:
         e = SimulationEngineHIP(0)
@@ -141,7 +143,7 @@ def main():
         f"{sum(s.n for s in scene_list)} particles, generated in {time.perf_counter() - t0:.1f} s")
 
     flags = ((SPH_FLAG_HIP_GRAPH if args.graph else 0) | (SPH_FLAG_CELL_WALK if args.walk == "cells" else 0)
-             | (SPH_FLAG_RUN_WALK if args.walk == "runs" else 0))
+             | (SPH_FLAG_RUN_WALK if args.walk == "runs" else 0) | (SPH_FLAG_TILE_WALK if args.walk == "tile" else 0))
     plain = args.config.endswith("-plain")
     engines = []
     if plain:

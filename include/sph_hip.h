@@ -44,8 +44,11 @@ extern "C" {
 #define SPH_FLAG_FP16_OFFSETS 0x4u  /* mixed precision: FP16 cell-centre offsets, Kahan density sums */
 #define SPH_FLAG_CELL_WALK 0x8u     /* neighbour passes read every candidate from HBM, one cell at a
                                        time: the reference's loop shape, kept for comparison */
-#define SPH_FLAG_RUN_WALK 0x10u     /* neighbour passes read candidates from HBM in merged Morton runs;
-                                       default (neither flag): LDS-staged bucket tiles */
+#define SPH_FLAG_RUN_WALK 0x10u     /* neighbour passes read candidates from HBM in merged Morton runs */
+#define SPH_FLAG_TILE_WALK 0x20u    /* neighbour passes stage each workgroup's buckets in LDS.
+                                       Default (no walk flag): cell-group waves, where a wave tests
+                                       the candidates of a cell against all of the cell's particles
+                                       and keeps a per-particle neighbour mask for the force pass */
 
 /* sph_validate result bits (the invariants of SimulationEngine.h:115-282). */
 #define SPH_V_NONFINITE 0x1u        /* NaN or Inf in position, velocity or pressure */

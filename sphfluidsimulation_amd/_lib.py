@@ -26,6 +26,7 @@ SPH_FLAG_RECORD_DEBUG = 0x2
 SPH_FLAG_FP16_OFFSETS = 0x4
 SPH_FLAG_CELL_WALK = 0x8
 SPH_FLAG_RUN_WALK = 0x10
+SPH_FLAG_TILE_WALK = 0x20
 
 SPH_V_NONFINITE = 0x1
 SPH_V_KEY = 0x2

@@ -1,0 +1,467 @@
+// Cell-group neighbour passes ("cell-wave walk", the default).
+//
+// Each wave owns 64 consecutive particles in bucket order. Consecutive
+// particles with the same cell code form a group; all particles of a group
+// share the same 27 neighbour buckets. For each group the wave:
+//   1. computes the 27 neighbour buckets on lanes 0..26, in the reference's
+//      x-y-z visit order (SF:397-404), with their particle ranges;
+//   2. lays the candidates out as four rounds of 64 lanes, round r holding
+//      buckets 8r .. 8r+7 with up to 8 particles each (lane = 8 * bucket + slot),
+//      so every lane loads one candidate and the loads are contiguous per bucket;
+//   3. tests every candidate against every target of the group, the target's
+//      position being wave-uniform: a handful of VALU operations per pair and
+//      no per-lane loop bookkeeping. Each (target, round) test is a 64-bit
+//      ballot, written into the target's lane.
+// After the groups, every lane holds its own particle's neighbour mask (4 x 64
+// bits over the candidate layout) and walks only the set bits. The density pass
+// stores the mask; the force pass reads it instead of testing again. Buckets
+// with more than 8 particles (crowded cells) leave their extra particles out of
+// the layout; each lane of such a group tests them directly, so every bucket is
+// visited exactly as often as in the reference, colliding buckets included.
+// SF = SPHFluidSimulation/source/SPH/Kernels/SPHFunctions.cpp.
+#pragma once
+
+#include "sph_kernels.hpp"
+
+namespace sph {
+
+constexpr uint32_t kWaves = kBlock / 64;  // waves per workgroup
+constexpr uint32_t kGroupBatch = 16;      // groups whose bucket tables a wave keeps at once
+constexpr uint32_t kCwMaskWords = 8;      // 4 rounds x 64 bits per particle
+constexpr uint32_t kSlots = 8;            // candidates per bucket in the round layout
+
+// Per-wave scratch: bucket start and count of the 27 neighbour buckets of up to
+// kGroupBatch groups.
+struct GroupTables {
+    uint32_t start[kWaves][kGroupBatch][27];
+    uint32_t cnt[kWaves][kGroupBatch][27];
+};
+
+__device__ inline uint64_t ballot64(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+// Lane t of m[r] takes the wave-uniform ballot b[r], r = 0..3 (v_writelane_b32
+// with the lane select in m0, since the data operand already uses the one scalar
+// operand a VALU instruction may read; the s_nop covers the wait states between
+// writing m0 and reading it as a lane select).
+__device__ inline void writelane4(uint64_t (&m)[4], const uint64_t (&b)[4], uint32_t t) {
+    uint32_t m0 = (uint32_t)m[0], m1 = (uint32_t)(m[0] >> 32), m2 = (uint32_t)m[1], m3 = (uint32_t)(m[1] >> 32);
+    uint32_t m4 = (uint32_t)m[2], m5 = (uint32_t)(m[2] >> 32), m6 = (uint32_t)m[3], m7 = (uint32_t)(m[3] >> 32);
+    asm volatile(
+        "s_nop 4\n\t"
+        "v_writelane_b32 %0, %8, m0\n\t"
+        "v_writelane_b32 %1, %9, m0\n\t"
+        "v_writelane_b32 %2, %10, m0\n\t"
+        "v_writelane_b32 %3, %11, m0\n\t"
+        "v_writelane_b32 %4, %12, m0\n\t"
+        "v_writelane_b32 %5, %13, m0\n\t"
+        "v_writelane_b32 %6, %14, m0\n\t"
+        "v_writelane_b32 %7, %15, m0"
+        : "+v"(m0), "+v"(m1), "+v"(m2), "+v"(m3), "+v"(m4), "+v"(m5), "+v"(m6), "+v"(m7)
+        : "s"((uint32_t)b[0]), "s"((uint32_t)(b[0] >> 32)), "s"((uint32_t)b[1]), "s"((uint32_t)(b[1] >> 32)),
+          "s"((uint32_t)b[2]), "s"((uint32_t)(b[2] >> 32)), "s"((uint32_t)b[3]), "s"((uint32_t)(b[3] >> 32)),
+          "{m0}"(t));
+    m[0] = ((uint64_t)m1 << 32) | m0;
+    m[1] = ((uint64_t)m3 << 32) | m2;
+    m[2] = ((uint64_t)m5 << 32) | m4;
+    m[3] = ((uint64_t)m7 << 32) | m6;
+}
+
+// m[r] without dynamic register indexing.
+__device__ inline uint64_t pick4(const uint64_t (&m)[4], uint32_t r) {
+    return r == 0u ? m[0] : r == 1u ? m[1] : r == 2u ? m[2] : m[3];
+}
+
+// Empty asm statements that take values as read-write register operands: every
+// load feeding them has to be issued and complete before this point, which keeps
+// the compiler from sinking a batch of loads into the predicated branches that
+// use them (that would turn one wait for four loads into four waits).
+__device__ inline void settle(uint32_t (&j)[4]) {
+    asm volatile("" : "+v"(j[0]), "+v"(j[1]), "+v"(j[2]), "+v"(j[3]));
+}
+__device__ inline void settle(float4& a) { asm volatile("" : "+v"(a.x), "+v"(a.y), "+v"(a.z), "+v"(a.w)); }
+
+// Walks the set bits of a neighbour mask four at a time: f(j) gets four
+// candidate indices (bucket start of the bit's bucket + slot), `self` standing in
+// for missing ones, so the four loads of a call are issued together. Rounds are
+// walked in order and bits in ascending order within a round.
+template <class F>
+__device__ inline void walk_mask(const uint64_t (&m)[4], const uint32_t* start, uint32_t self, F&& f) {
+#pragma unroll 1
+    for (uint32_t r = 0; r < 4; ++r) {
+        uint64_t mm = pick4(m, r);
+        const uint32_t* st = start + 8u * r;
+        while (mm) {
+            uint32_t l[4], b[4], j[4];
+            bool ok[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) {
+                ok[u] = mm != 0ull;
+                l[u] = ok[u] ? (uint32_t)__builtin_ctzll(mm) : 0u;
+                mm &= mm - 1ull;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) b[u] = st[l[u] >> 3];  // valid slots: read unconditionally
+            settle(b);
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) j[u] = ok[u] ? b[u] + (l[u] & 7u) : self;
+            f(j);
+        }
+    }
+}
+
+__device__ inline float readlanef(float v, uint32_t lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)lane));
+}
+
+// Lanes 0..26: the bucket range of neighbour cell `lane` (visit order x-y-z) of
+// the cell (gx, gy, gz). Other lanes get an empty range.
+__device__ inline void neighbour_bucket(uint32_t lane, uint32_t gx, uint32_t gy, uint32_t gz, const SceneDev& S,
+                                        const uint32_t* __restrict__ table, uint32_t lo, uint32_t hi,
+                                        uint32_t& start, uint32_t& cnt) {
+    start = lo;
+    cnt = 0u;
+    if (lane < 27u) {
+        const uint32_t a = lane / 9u, b = (lane / 3u) % 3u, c = lane % 3u;
+        const uint32_t code = morton3(gx + a - 1u, gy + b - 1u, gz + c - 1u);
+        const uint32_t bk = fastmod(code, S.magicM, S.M);
+        const uint32_t tb = table[bk], te = table[bk + 1];
+        const uint32_t s = tb > lo ? tb : lo, e = te < hi ? te : hi;
+        start = s;
+        cnt = e > s ? e - s : 0u;
+    }
+}
+
+// Candidate of round r on this lane: bucket 8r + lane/8, slot lane%8.
+struct RoundCand {
+    uint32_t j;
+    bool valid;
+};
+
+__device__ inline RoundCand round_candidate(uint32_t r, uint32_t lane, uint32_t start27, uint32_t cnt27) {
+    const uint32_t bi = 8u * r + (lane >> 3);
+    const int src = (int)(bi < 27u ? bi : 0u);
+    const uint32_t bs = (uint32_t)__shfl((int)start27, src);
+    const uint32_t bc = (uint32_t)__shfl((int)cnt27, src);
+    RoundCand c;
+    c.valid = bi < 27u && (lane & 7u) < bc;
+    c.j = bs + (lane & 7u);
+    return c;
+}
+
+// Iterates the groups of a wave in batches: phase1(slot, g0, g1, start27, cnt27)
+// runs wave-uniformly per group (lanes g0..g1-1 are its particles), then
+// phase2(slot, crowded) runs on the lanes of the batch's groups with `slot` their
+// group's table slot and `crowded` the bit set of its buckets holding more than
+// kSlots particles.
+template <class P1, class P2>
+__device__ inline void for_each_group(uint32_t lane, uint32_t wave, bool valid, uint32_t code, uint32_t cx,
+                                      uint32_t cy, uint32_t cz, const SceneDev& S, const uint32_t* __restrict__ table,
+                                      uint32_t lo, uint32_t hi, GroupTables& gt, P1&& phase1, P2&& phase2) {
+    const uint32_t prevCode = (uint32_t)__shfl_up((int)code, 1);
+    uint64_t heads = ballot64(valid && (lane == 0u || prevCode != code));
+    const uint64_t stops = heads | ~ballot64(valid);  // a group ends at the next head or invalid lane
+    while (heads) {
+        uint32_t mySlot = 0xFFFFFFFFu, myCrowded = 0u;
+        uint32_t slot = 0;
+        for (; heads && slot < kGroupBatch; ++slot) {
+            const uint32_t g0 = (uint32_t)__builtin_ctzll(heads);
+            heads &= heads - 1ull;
+            const uint64_t above = g0 < 63u ? (stops & ~((2ull << g0) - 1ull)) : 0ull;
+            const uint32_t g1 = above ? (uint32_t)__builtin_ctzll(above) : 64u;
+            const uint32_t gx = (uint32_t)__builtin_amdgcn_readlane((int)cx, (int)g0);
+            const uint32_t gy = (uint32_t)__builtin_amdgcn_readlane((int)cy, (int)g0);
+            const uint32_t gz = (uint32_t)__builtin_amdgcn_readlane((int)cz, (int)g0);
+            uint32_t start27, cnt27;
+            neighbour_bucket(lane, gx, gy, gz, S, table, lo, hi, start27, cnt27);
+            if (lane < 27u) {
+                gt.start[wave][slot][lane] = start27;
+                gt.cnt[wave][slot][lane] = cnt27;
+            }
+            const uint32_t crowded = (uint32_t)ballot64(cnt27 > kSlots);  // buckets 0..26
+            if (lane >= g0 && lane < g1) {
+                mySlot = slot;
+                myCrowded = crowded;
+            }
+            phase1(slot, g0, g1, start27, cnt27);
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (mySlot != 0xFFFFFFFFu) phase2(mySlot, myCrowded);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+template <bool kStatic, bool kDebug>
+__global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
+                                                       const SceneDev* __restrict__ scenes, float4* pos,
+                                                       const uint32_t* __restrict__ tab,
+                                                       const float4* __restrict__ spos,
+                                                       const uint32_t* __restrict__ stab,
+                                                       uint32_t* __restrict__ mask, float* __restrict__ dens) {
+    __shared__ GroupTables gt;
+    const uint32_t tile = xcd_tile(blockIdx.x, numBlocks);
+    if (tile >= numBlocks) return;
+    const BlockDesc bd = blocks[tile];
+    const SceneDev& S = scenes[bd.scene];
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    const uint32_t wbegin = bd.begin + 64u * wave;
+    if (wbegin >= bd.end) return;  // whole wave leaves; only wave-level sync below
+    const uint32_t wcount = min(64u, bd.end - wbegin);
+    const uint32_t i = wbegin + lane;
+    const bool valid = lane < wcount;
+    const float4 xi = pos[valid ? i : wbegin];
+    const float h = S.h, h2 = S.h2;
+    const float h2wide = h2 * 1.0001f;
+    const uint32_t cx = cell_coord(xi.x, h), cy = cell_coord(xi.y, h), cz = cell_coord(xi.z, h);
+    const uint32_t code = morton3(cx, cy, cz);
+    const uint32_t* table = tab + S.tBegin;
+    const uint32_t lo = S.pBegin, hi = S.pBegin + S.pCount;
+
+    float sumDyn = 0.f;
+    uint64_t m[4] = {0ull, 0ull, 0ull, 0ull};  // this lane's neighbour mask, rounds 0..3
+    // Exact contribution of candidates j (SF:426-453).
+    auto d0 = [&](uint32_t j) {
+        const float4 xj = pos[j];
+        const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 <= h2 && j != i) {
+            const float r = __builtin_amdgcn_sqrtf(r2);
+            const float q = h - r;
+            sumDyn += (r < h) ? q * q * q : 0.f;
+        }
+    };
+    auto d4 = [&](const uint32_t (&j)[4]) {
+        float4 xj[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) xj[u] = pos[j[u]];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) settle(xj[u]);
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            const float dx = xj[u].x - xi.x, dy = xj[u].y - xi.y, dz = xj[u].z - xi.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            const float r = __builtin_amdgcn_sqrtf(r2);
+            const float q = h - r;
+            sumDyn += (r2 <= h2 && j[u] != i && r < h) ? q * q * q : 0.f;
+        }
+    };
+    for_each_group(
+        lane, wave, valid, code, cx, cy, cz, S, table, lo, hi, gt,
+        [&](uint32_t, uint32_t g0, uint32_t g1, uint32_t start27, uint32_t cnt27) {
+            // The particle itself passes its own test (r = 0); d0 skips it. Empty
+            // slots are masked out with the (wave-uniform) slot masks.
+            float4 xc[4];
+            uint64_t occupied[4];
+#pragma unroll
+            for (uint32_t r = 0; r < 4; ++r) {
+                const RoundCand c = round_candidate(r, lane, start27, cnt27);
+                xc[r] = pos[c.valid ? c.j : lo];
+                occupied[r] = ballot64(c.valid);
+            }
+            for (uint32_t t = g0; t < g1; ++t) {
+                const float px = readlanef(xi.x, t), py = readlanef(xi.y, t), pz = readlanef(xi.z, t);
+                uint64_t b[4];
+#pragma unroll
+                for (uint32_t r = 0; r < 4; ++r) {
+                    const float dx = xc[r].x - px, dy = xc[r].y - py, dz = xc[r].z - pz;
+                    b[r] = ballot64(dx * dx + dy * dy + dz * dz <= h2wide) & occupied[r];
+                }
+                writelane4(m, b, t);
+            }
+        },
+        [&](uint32_t slot, uint32_t crowded) {
+            walk_mask(m, gt.start[wave][slot], i, d4);
+            // Crowded buckets: their particles past the kSlots-th.
+            for (; crowded; crowded &= crowded - 1u) {
+                const uint32_t bi = (uint32_t)__builtin_ctz(crowded);
+                const uint32_t s0 = gt.start[wave][slot][bi], n = gt.cnt[wave][slot][bi];
+                for (uint32_t t = kSlots; t < n; ++t) d0(s0 + t);
+            }
+        });
+    if (!valid) return;
+    float sumSt = 0.f;
+    if (kStatic && S.sCount) {
+        walk_runs(cx, cy, cz, S.Ms, S.magicMs, stab + S.stBegin, S.sBegin, S.sBegin + S.sCount,
+                  [&](uint32_t s, uint32_t e) {
+                      for (uint32_t j = s; j < e; ++j) {
+                          const float4 xj = spos[j];
+                          const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+                          const float r2 = dx * dx + dy * dy + dz * dz;
+                          if (r2 > h2) continue;
+                          const float r = __builtin_amdgcn_sqrtf(r2);
+                          const float q = h - r;
+                          sumSt += (r < h) ? q * q * q : 0.f;
+                      }
+                  });
+    }
+    const float rho = S.selfDensity + (sumDyn * S.mass + sumSt * S.mass) * S.C;
+    // Only this lane writes pos[i].w; no lane of this pass reads any .w.
+    reinterpret_cast<float*>(pos)[4ull * i + 3] = S.k * (rho - S.rho0);
+    if (kDebug) dens[i] = rho;
+    uint4* mrow = reinterpret_cast<uint4*>(mask + (size_t)i * kCwMaskWords);
+    mrow[0] = make_uint4((uint32_t)m[0], (uint32_t)(m[0] >> 32), (uint32_t)m[1], (uint32_t)(m[1] >> 32));
+    mrow[1] = make_uint4((uint32_t)m[2], (uint32_t)(m[2] >> 32), (uint32_t)m[3], (uint32_t)(m[3] >> 32));
+}
+
+template <bool kStatic, bool kDebug, bool kTri>
+__global__ __launch_bounds__(kBlock) void k_force_cw(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
+                                                     const SceneDev* __restrict__ scenes,
+                                                     const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                                     const uint32_t* __restrict__ tab,
+                                                     const float4* __restrict__ spos,
+                                                     const uint32_t* __restrict__ stab,
+                                                     const uint32_t* __restrict__ mask,
+                                                     const uint32_t* __restrict__ ids, const float* __restrict__ tris,
+                                                     float dt, float4* __restrict__ posT, float4* __restrict__ velT,
+                                                     uint32_t* __restrict__ counts, float4* __restrict__ accOut) {
+    __shared__ GroupTables gt;
+    const uint32_t tile = xcd_tile(blockIdx.x, numBlocks);
+    if (tile >= numBlocks) return;
+    const BlockDesc bd = blocks[tile];
+    const SceneDev& S = scenes[bd.scene];
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    const uint32_t wbegin = bd.begin + 64u * wave;
+    if (wbegin >= bd.end) return;
+    const uint32_t wcount = min(64u, bd.end - wbegin);
+    const uint32_t i = wbegin + lane;
+    const bool valid = lane < wcount;
+    const uint32_t iv = valid ? i : wbegin;
+    const float4 xi = pos[iv];
+    const float4 vi = vel[iv];
+    const float pi = xi.w;  // this step's pressure (density pass)
+    const float h = S.h, h2 = S.h2;
+    const uint32_t cx = cell_coord(xi.x, h), cy = cell_coord(xi.y, h), cz = cell_coord(xi.z, h);
+    const uint32_t code = morton3(cx, cy, cz);
+    const uint32_t* table = tab + S.tBegin;
+    const uint32_t lo = S.pBegin, hi = S.pBegin + S.pCount;
+    uint64_t m[4];
+    {
+        const uint4* mr = reinterpret_cast<const uint4*>(mask + (size_t)iv * kCwMaskWords);
+        const uint4 a = mr[0], b = mr[1];
+        m[0] = ((uint64_t)a.y << 32) | a.x;
+        m[1] = ((uint64_t)a.w << 32) | a.z;
+        m[2] = ((uint64_t)b.y << 32) | b.x;
+        m[3] = ((uint64_t)b.w << 32) | b.z;
+    }
+    const float rho = pi / S.k + S.rho0;  // SF:544
+    const float absP = fabsf(pi);
+    float fpx = 0.f, fpy = 0.f, fpz = 0.f;
+    float fvx = 0.f, fvy = 0.f, fvz = 0.f;
+    float fsx = 0.f, fsy = 0.f, fsz = 0.f;
+    float sumD2s = 0.f;
+    // Pressure weights of coincident pairs (r == 0): their direction is the
+    // particle's random direction (SF:566-570), the same for every such pair, so
+    // it is evaluated once after the walks.
+    float zeroSp = 0.f;
+    bool anyZero = false;
+    // One dynamic candidate j (SF:557-589), with its position (pressure in w)
+    // and velocity already loaded.
+    auto pair_at = [&](uint32_t j, const float4 xj, const float4 vj) {
+        const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const bool ok = j != i && r2 <= h2;
+        const float r = __builtin_amdgcn_sqrtf(r2);
+        const bool zero = r2 == 0.f || r == 0.f;
+        const float inv = zero ? 0.f : __builtin_amdgcn_rcpf(r);
+        const float q = h - r;
+        const bool in = r < h;
+        const float w1 = in ? -3.0f * q * q : 0.f;
+        const float w2 = in ? 6.0f * q : 0.f;
+        const float sp = (pi + xj.w) * w1;
+        fpx += ok ? (dx * inv) * sp : 0.f;
+        fpy += ok ? (dy * inv) * sp : 0.f;
+        fpz += ok ? (dz * inv) * sp : 0.f;
+        zeroSp += ok && zero ? sp : 0.f;
+        anyZero |= ok && zero;
+        fvx += ok ? (vj.x - vi.x) * w2 : 0.f;
+        fvy += ok ? (vj.y - vi.y) * w2 : 0.f;
+        fvz += ok ? (vj.z - vi.z) * w2 : 0.f;
+    };
+    auto pair = [&](uint32_t j) { pair_at(j, pos[j], vel[j]); };
+    auto pair4 = [&](const uint32_t (&j)[4]) {
+        float4 xj[4], vj[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            xj[u] = pos[j[u]];
+            vj[u] = vel[j[u]];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            settle(xj[u]);
+            settle(vj[u]);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) pair_at(j[u], xj[u], vj[u]);
+    };
+    for_each_group(
+        lane, wave, valid, code, cx, cy, cz, S, table, lo, hi, gt,
+        [&](uint32_t, uint32_t, uint32_t, uint32_t, uint32_t) {},
+        [&](uint32_t slot, uint32_t crowded) {
+            walk_mask(m, gt.start[wave][slot], i, pair4);
+            // Crowded buckets: their particles past the kSlots-th.
+            for (; crowded; crowded &= crowded - 1u) {
+                const uint32_t bi = (uint32_t)__builtin_ctz(crowded);
+                const uint32_t s0 = gt.start[wave][slot][bi], n = gt.cnt[wave][slot][bi];
+                for (uint32_t t = kSlots; t < n; ++t) pair(s0 + t);
+            }
+        });
+    if (!valid) return;
+    if (kStatic && S.sCount) {
+        walk_runs(cx, cy, cz, S.Ms, S.magicMs, stab + S.stBegin, S.sBegin, S.sBegin + S.sCount,
+                  [&](uint32_t s, uint32_t e) {
+                      for (uint32_t j = s; j < e; ++j) {
+                          const float4 xj = spos[j];
+                          const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+                          const float r2 = dx * dx + dy * dy + dz * dz;
+                          if (r2 > h2) continue;
+                          const float r = __builtin_amdgcn_sqrtf(r2);
+                          const bool zero = r2 == 0.f || r == 0.f;
+                          const float inv = zero ? 0.f : __builtin_amdgcn_rcpf(r);
+                          const float q = h - r;
+                          const bool in = r < h;
+                          const float sp = absP * (in ? -3.0f * q * q : 0.f);
+                          fsx += (dx * inv) * sp;
+                          fsy += (dy * inv) * sp;
+                          fsz += (dz * inv) * sp;
+                          zeroSp += zero ? sp : 0.f;
+                          anyZero |= zero;
+                          sumD2s += in ? 6.0f * q : 0.f;
+                      }
+                  });
+    }
+    if (anyZero) {
+        const float3 d = random_direction((float)(ids[i] - S.pBegin));
+        fpx += d.x * zeroSp;
+        fpy += d.y * zeroSp;
+        fpz += d.z * zeroSp;
+    }
+    // SF:626-638
+    const float sP = S.mass / (2.0f * rho) * S.C;
+    const float sV = S.mu * S.mass * S.C;
+    const float sVs = S.mu * S.mass * S.C * 0.0f;
+    float Fx = fpx * sP + fvx * sV + fsx * sP + (-vi.x * sumD2s) * sVs;
+    float Fy = fpy * sP + fvy * sV + fsy * sP + (-vi.y * sumD2s) * sVs;
+    float Fz = fpz * sP + fvz * sV + fsz * sP + (-vi.z * sumD2s) * sVs;
+    if (kTri) {
+        const float3 p3 = make_float3(xi.x, xi.y, xi.z);
+        for (uint32_t t = 0; t < S.triCount; ++t) {
+            const float3 f = triangle_force(p3, tris + 9u * (S.triBegin + t));
+            Fx += f.x;
+            Fy += f.y;
+            Fz += f.z;
+        }
+    }
+    const float ax = Fx / rho + S.gx, ay = Fy / rho + S.gy, az = Fz / rho + S.gz;
+    if (kDebug) accOut[i] = make_float4(ax, ay, az, 0.f);
+    // SF:641-645: symplectic Euler, then the key of the new cell.
+    const float nvx = vi.x + ax * dt, nvy = vi.y + ay * dt, nvz = vi.z + az * dt;
+    const float nx = xi.x + nvx * dt, ny = xi.y + nvy * dt, nz = xi.z + nvz * dt;
+    const uint32_t ncode = morton3(cell_coord(nx, h), cell_coord(ny, h), cell_coord(nz, h));
+    const uint32_t key = fastmod(ncode, S.magicM, S.M);
+    posT[i] = make_float4(nx, ny, nz, pi);
+    velT[i] = make_float4(nvx, nvy, nvz, __uint_as_float(key));
+    const uint32_t gk = S.tBegin + key;
+    const Run run = wave_run(gk);
+    if (run.is_head) atomicAdd(&counts[gk], run.len);
+}
+
+}  // namespace sph

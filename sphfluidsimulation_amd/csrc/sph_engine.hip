@@ -18,6 +18,7 @@
 #include "../../include/sph_hip.h"
 #include "sph_kernels.hpp"
 #include "sph_tile.hpp"
+#include "sph_cellwave.hpp"
 
 using namespace sph;
 
@@ -214,6 +215,16 @@ int collect_events(sph_engine* h) {
     return SPH_OK;
 }
 
+// Neighbour-pass variant chosen by the initialize flags.
+enum class Walk { CellWave, Runs, Tiles, Cells };
+
+Walk walk_of(uint32_t flags) {
+    if (flags & SPH_FLAG_CELL_WALK) return Walk::Cells;
+    if (flags & SPH_FLAG_RUN_WALK) return Walk::Runs;
+    if (flags & SPH_FLAG_TILE_WALK) return Walk::Tiles;
+    return Walk::CellWave;
+}
+
 // Enqueues one step starting from parity h->cur (does not flip it).
 int enqueue_step(sph_engine* h, float dt, bool capture) {
     const int c = h->cur;
@@ -245,12 +256,13 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     int rc;
     // Density / pressure.
     if ((rc = begin(SPH_STAGE_DENSITY))) return rc;
-    const bool cells = (h->flags & SPH_FLAG_CELL_WALK) != 0;
-    const bool runs = !cells && (h->flags & SPH_FLAG_RUN_WALK) != 0;
-    const bool tiles = !cells && !runs;
-    if (!cells) {
+    const Walk walk = walk_of(h->flags);
+    if (walk != Walk::Cells) {
 #define SPH_DENS(S, D)                                                                                         \
-    if (tiles)                                                                                                 \
+    if (walk == Walk::CellWave)                                                                                \
+        hipLaunchKernelGGL((k_density_cw<S, D>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,         \
+                           h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->mask, h->dens);                 \
+    else if (walk == Walk::Tiles)                                                                              \
         hipLaunchKernelGGL((k_density_tile<S, D>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,       \
                            h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->mask, h->dens);                 \
     else                                                                                                       \
@@ -292,11 +304,15 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     if ((rc = begin(SPH_STAGE_FORCE))) return rc;
 #define SPH_FORCE(S, D, T)                                                                                        \
     do {                                                                                                          \
-    if (tiles)                                                                                                    \
+    if (walk == Walk::CellWave)                                                                                   \
+        hipLaunchKernelGGL((k_force_cw<S, D, T>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,           \
+                           h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->mask, h->ids[c], h->tris,  \
+                           dt, h->posT, h->velT, h->tab[c ^ 1], h->acc);                                          \
+    else if (walk == Walk::Tiles)                                                                                 \
         hipLaunchKernelGGL((k_force_tile<S, D, T>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,         \
                            h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->mask, h->ids[c], h->tris,  \
                            dt, h->posT, h->velT, h->tab[c ^ 1], h->acc);                                          \
-    else if (runs)                                                                                                \
+    else if (walk == Walk::Runs)                                                                                  \
         hipLaunchKernelGGL((k_force_runs<S, D, T>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,        \
                            h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->mask, h->ids[c], h->tris,  \
                            dt, h->posT, h->velT, h->tab[c ^ 1], h->acc);                                          \
@@ -567,7 +583,8 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
     SPH_ALLOC(h->consOut, 8ull * n_scenes);
     SPH_ALLOC(h->valBits, 1);
     SPH_ALLOC(h->valIdx, 8ull * n_scenes);
-    static_assert(kMaskWords == kTileMaskWords, "both mask formats use 8 words per particle");
+    static_assert(kMaskWords == kTileMaskWords && kMaskWords == kCwMaskWords,
+                  "every mask format uses 8 words per particle");
     if (!(flags & SPH_FLAG_CELL_WALK)) SPH_ALLOC(h->mask, kMaskWords * std::max<uint64_t>(N, 1));  // tile and run walks
     if (flags & SPH_FLAG_RECORD_DEBUG) {
         SPH_ALLOC(h->dens, std::max<uint64_t>(N, 1));

@@ -170,11 +170,6 @@ __device__ inline void stage_tile(BucketMap& m, uint32_t* s_code, uint32_t* s_w,
 // to 8 words per particle.
 constexpr uint32_t kTileMaskWords = 8;
 
-// Staged-or-HBM read of candidate t of a bucket view.
-__device__ inline float4 cand_pos(const BucketView& v, uint32_t t, const float4* candP, const float4* pos) {
-    return v.off != kNotStaged ? candP[v.off + t] : pos[v.beg + t];
-}
-
 template <bool kStatic, bool kDebug>
 __global__ __launch_bounds__(kBlock) void k_density_tile(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
                                                          const SceneDev* __restrict__ scenes, float4* pos,
@@ -206,16 +201,39 @@ __global__ __launch_bounds__(kBlock) void k_density_tile(const BlockDesc* __rest
     if (!valid) return;
 
     float sumDyn = 0.f, sumSt = 0.f;
-    // Exact in-range contribution of candidate t of bucket v (SF:426-453).
-    auto d0 = [&](const BucketView& v, uint32_t t) {
-        const float4 xj = cand_pos(v, t, candP, pos);
-        const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 <= h2 && v.beg + t != i) {
-            const float r = __builtin_amdgcn_sqrtf(r2);
-            const float q = h - r;
-            sumDyn += (r < h) ? q * q * q : 0.f;
+    // The candidates of one bucket visit, read through `src` (the LDS copy or
+    // HBM): widened test of the first 8 (four loads in flight), then the exact
+    // contribution (SF:426-453) of the flagged ones, then any past the 8th.
+    // Returns the visit's mask byte.
+    auto visit_bucket = [&](const float4* src, uint32_t beg, uint32_t cnt) -> uint32_t {
+        const uint32_t n8 = cnt < 8u ? cnt : 8u;
+        const uint32_t selfT = i - beg;  // >= cnt unless i is in this bucket
+        uint32_t bits8 = 0;
+        for (uint32_t t = 0; t < n8; t += 4u) {
+            float4 xs[4];
+#pragma unroll
+            for (uint32_t q = 0; q < 4; ++q) xs[q] = src[min(t + q, n8 - 1u)];
+#pragma unroll
+            for (uint32_t q = 0; q < 4; ++q) {
+                const float dx = xs[q].x - xi.x, dy = xs[q].y - xi.y, dz = xs[q].z - xi.z;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                const bool in = (t + q < n8) && (t + q != selfT) && r2 <= h2wide;
+                bits8 |= in ? (1u << (t + q)) : 0u;
+            }
         }
+        auto d0 = [&](uint32_t t) {
+            const float4 xj = src[t];
+            const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 <= h2 && t != selfT) {
+                const float r = __builtin_amdgcn_sqrtf(r2);
+                const float q = h - r;
+                sumDyn += (r < h) ? q * q * q : 0.f;
+            }
+        };
+        for (uint32_t mb = bits8; mb; mb &= mb - 1u) d0((uint32_t)__builtin_ctz(mb));
+        for (uint32_t t = 8; t < cnt; ++t) d0(t);
+        return bits8;
     };
     uint32_t word = 0;
     for (int a = 0; a < 3; ++a)
@@ -224,26 +242,8 @@ __global__ __launch_bounds__(kBlock) void k_density_tile(const BlockDesc* __rest
                 const uint32_t visit = (uint32_t)(a * 9 + b * 3 + c);
                 const uint32_t bucket = fastmod(cell_code(cs, a, b, c), S.magicM, S.M);
                 const BucketView v = bucket_view(m, bucket, table, lo, hi);
-                const uint32_t n8 = v.cnt < 8u ? v.cnt : 8u;
-                const uint32_t selfT = i - v.beg;  // >= cnt unless i is in this bucket
-                // First 8 candidates: widened test only, four loads in flight.
-                uint32_t bits8 = 0;
-                for (uint32_t t = 0; t < n8; t += 4u) {
-                    float4 xs[4];
-#pragma unroll
-                    for (uint32_t q = 0; q < 4; ++q) xs[q] = cand_pos(v, min(t + q, n8 - 1u), candP, pos);
-#pragma unroll
-                    for (uint32_t q = 0; q < 4; ++q) {
-                        const float dx = xs[q].x - xi.x, dy = xs[q].y - xi.y, dz = xs[q].z - xi.z;
-                        const float r2 = dx * dx + dy * dy + dz * dz;
-                        const bool in = (t + q < n8) && (t + q != selfT) && r2 <= h2wide;
-                        bits8 |= in ? (1u << (t + q)) : 0u;
-                    }
-                }
-                // Then the exact contribution of the flagged ones.
-                for (uint32_t mb = bits8; mb; mb &= mb - 1u) d0(v, (uint32_t)__builtin_ctz(mb));
-                // Candidates past the 8th (crowded buckets) are handled directly.
-                for (uint32_t t = 8; t < v.cnt; ++t) d0(v, t);
+                const uint32_t bits8 = v.off != kNotStaged ? visit_bucket(candP + v.off, v.beg, v.cnt)
+                                                           : visit_bucket(pos + v.beg, v.beg, v.cnt);
                 word |= bits8 << (8u * (visit & 3u));
                 if ((visit & 3u) == 3u || visit == 26u) {
                     smask[(visit >> 2) * kBlock + tid] = word;
@@ -327,14 +327,13 @@ __global__ __launch_bounds__(kBlock) void k_force_tile(const BlockDesc* __restri
     float fvx = 0.f, fvy = 0.f, fvz = 0.f;
     float fsx = 0.f, fsy = 0.f, fsz = 0.f;
     float sumD2s = 0.f;
-    // One dynamic candidate t of bucket v (SF:557-589).
-    auto pair = [&](const BucketView& v, uint32_t t) {
-        const bool staged = v.off != kNotStaged;
-        const float4 xj = staged ? candP[v.off + t] : pos[v.beg + t];
+    // One dynamic candidate (position xj with this step's pressure in w,
+    // velocity from vsrc, global index j) of the walk (SF:557-589).
+    auto pair = [&](uint32_t j, const float4& xj, const float4* vsrc) {
         float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
         const float r2 = dx * dx + dy * dy + dz * dz;
-        if (v.beg + t == i || r2 > h2) return;
-        const float4 vj = staged ? candV[v.off + t] : vel[v.beg + t];
+        if (j == i || r2 > h2) return;
+        const float4 vj = *vsrc;
         const float r = __builtin_amdgcn_sqrtf(r2);
         if (r2 == 0.f || r == 0.f) {
             const float3 d = random_direction((float)(ids[i] - S.pBegin));
@@ -359,6 +358,15 @@ __global__ __launch_bounds__(kBlock) void k_force_tile(const BlockDesc* __restri
         fvy += (vj.y - vi.y) * w2;
         fvz += (vj.z - vi.z) * w2;
     };
+    // The recorded neighbours of one bucket visit, read through psrc/vsrc (LDS
+    // copies or HBM), then any candidate past the 8th.
+    auto visit_bucket = [&](const float4* psrc, const float4* vsrc, uint32_t beg, uint32_t cnt, uint32_t bits8) {
+        for (uint32_t mb = bits8; mb; mb &= mb - 1u) {
+            const uint32_t t = (uint32_t)__builtin_ctz(mb);
+            pair(beg + t, psrc[t], vsrc + t);
+        }
+        for (uint32_t t = 8; t < cnt; ++t) pair(beg + t, psrc[t], vsrc + t);
+    };
     for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b)
             for (int c = 0; c < 3; ++c) {
@@ -366,8 +374,10 @@ __global__ __launch_bounds__(kBlock) void k_force_tile(const BlockDesc* __restri
                 const uint32_t bucket = fastmod(cell_code(cs, a, b, c), S.magicM, S.M);
                 const BucketView v = bucket_view(m, bucket, table, lo, hi);
                 const uint32_t bits8 = (smask[(visit >> 2) * kBlock + tid] >> (8u * (visit & 3u))) & 0xFFu;
-                for (uint32_t mb = bits8; mb; mb &= mb - 1u) pair(v, (uint32_t)__builtin_ctz(mb));
-                for (uint32_t t = 8; t < v.cnt; ++t) pair(v, t);
+                if (v.off != kNotStaged)
+                    visit_bucket(candP + v.off, candV + v.off, v.beg, v.cnt, bits8);
+                else
+                    visit_bucket(pos + v.beg, vel + v.beg, v.beg, v.cnt, bits8);
             }
     if (kStatic && S.sCount) {
         walk_runs(cx, cy, cz, S.Ms, S.magicMs, stab + S.stBegin, S.sBegin, S.sBegin + S.sCount,

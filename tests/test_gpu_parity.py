@@ -20,7 +20,7 @@ import oracle as orc
 from helpers import bucket_multisets_equal, check_buckets, expected_starts, max_norm_rel, max_rel
 from sphfluidsimulation_amd import SPHError, SimulationEngineHIP, scenes
 from sphfluidsimulation_amd._lib import (SPH_E_STATE, SPH_FLAG_CELL_WALK, SPH_FLAG_HIP_GRAPH, SPH_FLAG_RECORD_DEBUG,
-                                         SPH_FLAG_RUN_WALK)
+                                         SPH_FLAG_RUN_WALK, SPH_FLAG_TILE_WALK)
 
 pytestmark = pytest.mark.gpu
 
@@ -30,7 +30,8 @@ FP64_DENS_TOL = 1e-4
 
 # The three neighbour-pass variants: LDS-staged bucket tiles (default), Morton
 # runs from HBM, and the reference's cell-by-cell loop from HBM.
-WALKS = [pytest.param(0, id="tile"), pytest.param(SPH_FLAG_RUN_WALK, id="runs"),
+WALKS = [pytest.param(0, id="cellwave"), pytest.param(SPH_FLAG_TILE_WALK, id="tile"),
+         pytest.param(SPH_FLAG_RUN_WALK, id="runs"),
          pytest.param(SPH_FLAG_CELL_WALK, id="cells")]
 
 
@@ -274,6 +275,28 @@ def test_debug_smallest_scene(walk):
     _init_parity(sc)
     got, ref = _one_step(sc, 1e-2, threads=1, walk=walk)
     _check_step(sc, got, ref, 1e-2)
+
+
+def _occupancy_scene(kind):
+    # crowded: 27 particles per cell on average, so every bucket holds more than
+    # the 8 candidates per bucket the cell-group layout keeps in registers;
+    # sparse: about 2 per cell, so a wave of 64 particles spans more than the 32
+    # cell groups whose bucket tables it keeps at once.
+    if kind == "crowded":
+        return scenes.dam_break(24, 24, 24, (2.4, 2.4, 2.4), spacing=0.1 / 3.0, jitter_frac=0.3, seed=11,
+                                shuffle=True, name="crowded-cells")
+    return scenes.dam_break(30, 30, 30, (4.0, 4.0, 4.0), spacing=0.08, jitter_frac=0.2, seed=12, shuffle=True,
+                            name="sparse-cells")
+
+
+@pytest.mark.parametrize("kind", ["crowded", "sparse"])
+@pytest.mark.parametrize("walk", WALKS)
+def test_cell_occupancy_extremes(kind, walk):
+    say(f"{kind} cells (walk flags {walk})")
+    sc = _occupancy_scene(kind)
+    _init_parity(sc)
+    got, ref = _one_step(sc, 1e-3, threads=4, walk=walk)
+    _check_step(sc, got, ref, 1e-3)
 
 
 def test_update_before_initialize_and_empty():
