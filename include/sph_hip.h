@@ -112,7 +112,9 @@ int sph_initialize(sph_handle h, const sph_params* params, const float* dyn_pos,
  * engine): n_scenes independent scenes advance together, one launch per stage.
  * dyn_offsets and static_offsets hold n_scenes + 1 prefix offsets into the
  * concatenated dyn_pos/dyn_vel and static_pos arrays; params holds one
- * sph_params per scene. Each scene keeps its own table of 2*N_s + 1 entries. */
+ * sph_params per scene. Each scene keeps its own table of 2*N_s + 1 entries.
+ * Both initialize calls accept up to 2^28 - 1 dynamic and 2^28 - 1 static
+ * particles in total (device offsets of 16-byte records are 32-bit). */
 int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* params, const uint64_t* dyn_offsets,
                          const float* dyn_pos, const float* dyn_vel, const uint64_t* static_offsets,
                          const float* static_pos, uint32_t flags);

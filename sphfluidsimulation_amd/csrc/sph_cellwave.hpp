@@ -18,6 +18,8 @@
 // with more than 8 particles (crowded cells) leave their extra particles out of
 // the layout; each lane of such a group tests them directly, so every bucket is
 // visited exactly as often as in the reference, colliding buckets included.
+// Static (wall) neighbours are walked per particle through merged Morton runs
+// of the static table afterwards.
 // SF = SPHFluidSimulation/source/SPH/Kernels/SPHFunctions.cpp.
 #pragma once
 
@@ -66,19 +68,25 @@ __device__ inline void writelane4(uint64_t (&m)[4], const uint64_t (&b)[4], uint
     m[3] = ((uint64_t)m7 << 32) | m6;
 }
 
+// Record j of an array of 16-byte records through a 32-bit byte offset (the
+// engine holds fewer than 2^28 particles of each kind), so the load takes a
+// scalar base and one 32-bit vector offset instead of 64-bit address arithmetic.
+__device__ inline float4 ld16(const float4* __restrict__ base, uint32_t j) {
+    return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) + (j << 4));
+}
+
 // m[r] without dynamic register indexing.
 __device__ inline uint64_t pick4(const uint64_t (&m)[4], uint32_t r) {
     return r == 0u ? m[0] : r == 1u ? m[1] : r == 2u ? m[2] : m[3];
 }
 
-// Empty asm statements that take values as read-write register operands: every
-// load feeding them has to be issued and complete before this point, which keeps
+// An empty asm statement that takes the values as read-write register operands:
+// the loads feeding them are issued and complete before this point, which keeps
 // the compiler from sinking a batch of loads into the predicated branches that
-// use them (that would turn one wait for four loads into four waits).
+// use them (one wait for four loads would become four waits).
 __device__ inline void settle(uint32_t (&j)[4]) {
     asm volatile("" : "+v"(j[0]), "+v"(j[1]), "+v"(j[2]), "+v"(j[3]));
 }
-__device__ inline void settle(float4& a) { asm volatile("" : "+v"(a.x), "+v"(a.y), "+v"(a.z), "+v"(a.w)); }
 
 // Walks the set bits of a neighbour mask four at a time: f(j) gets four
 // candidate indices (bucket start of the bit's bucket + slot), `self` standing in
@@ -113,24 +121,6 @@ __device__ inline float readlanef(float v, uint32_t lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)lane));
 }
 
-// Lanes 0..26: the bucket range of neighbour cell `lane` (visit order x-y-z) of
-// the cell (gx, gy, gz). Other lanes get an empty range.
-__device__ inline void neighbour_bucket(uint32_t lane, uint32_t gx, uint32_t gy, uint32_t gz, const SceneDev& S,
-                                        const uint32_t* __restrict__ table, uint32_t lo, uint32_t hi,
-                                        uint32_t& start, uint32_t& cnt) {
-    start = lo;
-    cnt = 0u;
-    if (lane < 27u) {
-        const uint32_t a = lane / 9u, b = (lane / 3u) % 3u, c = lane % 3u;
-        const uint32_t code = morton3(gx + a - 1u, gy + b - 1u, gz + c - 1u);
-        const uint32_t bk = fastmod(code, S.magicM, S.M);
-        const uint32_t tb = table[bk], te = table[bk + 1];
-        const uint32_t s = tb > lo ? tb : lo, e = te < hi ? te : hi;
-        start = s;
-        cnt = e > s ? e - s : 0u;
-    }
-}
-
 // Candidate of round r on this lane: bucket 8r + lane/8, slot lane%8.
 struct RoundCand {
     uint32_t j;
@@ -150,9 +140,9 @@ __device__ inline RoundCand round_candidate(uint32_t r, uint32_t lane, uint32_t 
 
 // Iterates the groups of a wave in batches: phase1(slot, g0, g1, start27, cnt27)
 // runs wave-uniformly per group (lanes g0..g1-1 are its particles), then
-// phase2(slot, crowded) runs on the lanes of the batch's groups with `slot` their
-// group's table slot and `crowded` the bit set of its buckets holding more than
-// kSlots particles.
+// phase2(slot, crowded) runs on the lanes of the batch's groups with `slot`
+// their group's table slot and `crowded` the bit set of its buckets holding more
+// than kSlots particles.
 template <class P1, class P2>
 __device__ inline void for_each_group(uint32_t lane, uint32_t wave, bool valid, uint32_t code, uint32_t cx,
                                       uint32_t cy, uint32_t cz, const SceneDev& S, const uint32_t* __restrict__ table,
@@ -171,9 +161,17 @@ __device__ inline void for_each_group(uint32_t lane, uint32_t wave, bool valid, 
             const uint32_t gx = (uint32_t)__builtin_amdgcn_readlane((int)cx, (int)g0);
             const uint32_t gy = (uint32_t)__builtin_amdgcn_readlane((int)cy, (int)g0);
             const uint32_t gz = (uint32_t)__builtin_amdgcn_readlane((int)cz, (int)g0);
-            uint32_t start27, cnt27;
-            neighbour_bucket(lane, gx, gy, gz, S, table, lo, hi, start27, cnt27);
+            // Lanes 0..26: neighbour cell `lane` in the reference's x-y-z visit
+            // order (SF:397-404) and its bucket ranges.
+            uint32_t start27 = lo, cnt27 = 0u;
             if (lane < 27u) {
+                const uint32_t a = lane / 9u, b = (lane / 3u) % 3u, c = lane % 3u;
+                const uint32_t ncode = morton3(gx + a - 1u, gy + b - 1u, gz + c - 1u);
+                const uint32_t bk = fastmod(ncode, S.magicM, S.M);
+                const uint32_t tb = table[bk], te = table[bk + 1];
+                const uint32_t s0 = tb > lo ? tb : lo, e0 = te < hi ? te : hi;
+                start27 = s0;
+                cnt27 = e0 > s0 ? e0 - s0 : 0u;
                 gt.start[wave][slot][lane] = start27;
                 gt.cnt[wave][slot][lane] = cnt27;
             }
@@ -208,33 +206,22 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
     const uint32_t wcount = min(64u, bd.end - wbegin);
     const uint32_t i = wbegin + lane;
     const bool valid = lane < wcount;
-    const float4 xi = pos[valid ? i : wbegin];
+    const float4 xi = ld16(pos, valid ? i : wbegin);
     const float h = S.h, h2 = S.h2;
     const float h2wide = h2 * 1.0001f;
     const uint32_t cx = cell_coord(xi.x, h), cy = cell_coord(xi.y, h), cz = cell_coord(xi.z, h);
     const uint32_t code = morton3(cx, cy, cz);
     const uint32_t* table = tab + S.tBegin;
+    const uint32_t* stable = stab + S.stBegin;
     const uint32_t lo = S.pBegin, hi = S.pBegin + S.pCount;
 
-    float sumDyn = 0.f;
+    float sumDyn = 0.f, sumSt = 0.f;
     uint64_t m[4] = {0ull, 0ull, 0ull, 0ull};  // this lane's neighbour mask, rounds 0..3
-    // Exact contribution of candidates j (SF:426-453).
-    auto d0 = [&](uint32_t j) {
-        const float4 xj = pos[j];
-        const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 <= h2 && j != i) {
-            const float r = __builtin_amdgcn_sqrtf(r2);
-            const float q = h - r;
-            sumDyn += (r < h) ? q * q * q : 0.f;
-        }
-    };
+    // Exact contributions of four candidates j (SF:426-453).
     auto d4 = [&](const uint32_t (&j)[4]) {
         float4 xj[4];
 #pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) xj[u] = pos[j[u]];
-#pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) settle(xj[u]);
+        for (uint32_t u = 0; u < 4; ++u) xj[u] = ld16(pos, j[u]);
 #pragma unroll
         for (uint32_t u = 0; u < 4; ++u) {
             const float dx = xj[u].x - xi.x, dy = xj[u].y - xi.y, dz = xj[u].z - xi.z;
@@ -244,17 +231,28 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
             sumDyn += (r2 <= h2 && j[u] != i && r < h) ? q * q * q : 0.f;
         }
     };
+    // Static neighbours (SF:455-496).
+    auto dWall = [&](uint32_t j) {
+        const float4 xj = ld16(spos, j);
+        const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 <= h2) {
+            const float r = __builtin_amdgcn_sqrtf(r2);
+            const float q = h - r;
+            sumSt += (r < h) ? q * q * q : 0.f;
+        }
+    };
     for_each_group(
         lane, wave, valid, code, cx, cy, cz, S, table, lo, hi, gt,
         [&](uint32_t, uint32_t g0, uint32_t g1, uint32_t start27, uint32_t cnt27) {
-            // The particle itself passes its own test (r = 0); d0 skips it. Empty
+            // The particle itself passes its own test (r = 0); d4 skips it. Empty
             // slots are masked out with the (wave-uniform) slot masks.
             float4 xc[4];
             uint64_t occupied[4];
 #pragma unroll
             for (uint32_t r = 0; r < 4; ++r) {
                 const RoundCand c = round_candidate(r, lane, start27, cnt27);
-                xc[r] = pos[c.valid ? c.j : lo];
+                xc[r] = ld16(pos, c.valid ? c.j : lo);
                 occupied[r] = ballot64(c.valid);
             }
             for (uint32_t t = g0; t < g1; ++t) {
@@ -270,29 +268,23 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
         },
         [&](uint32_t slot, uint32_t crowded) {
             walk_mask(m, gt.start[wave][slot], i, d4);
-            // Crowded buckets: their particles past the kSlots-th.
+            // Crowded buckets: their particles past the kSlots-th, four at a time.
             for (; crowded; crowded &= crowded - 1u) {
                 const uint32_t bi = (uint32_t)__builtin_ctz(crowded);
                 const uint32_t s0 = gt.start[wave][slot][bi], n = gt.cnt[wave][slot][bi];
-                for (uint32_t t = kSlots; t < n; ++t) d0(s0 + t);
+                for (uint32_t t = kSlots; t < n; t += 4u) {
+                    uint32_t j[4];
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u) j[u] = t + u < n ? s0 + t + u : i;
+                    d4(j);
+                }
             }
         });
     if (!valid) return;
-    float sumSt = 0.f;
-    if (kStatic && S.sCount) {
-        walk_runs(cx, cy, cz, S.Ms, S.magicMs, stab + S.stBegin, S.sBegin, S.sBegin + S.sCount,
-                  [&](uint32_t s, uint32_t e) {
-                      for (uint32_t j = s; j < e; ++j) {
-                          const float4 xj = spos[j];
-                          const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-                          const float r2 = dx * dx + dy * dy + dz * dz;
-                          if (r2 > h2) continue;
-                          const float r = __builtin_amdgcn_sqrtf(r2);
-                          const float q = h - r;
-                          sumSt += (r < h) ? q * q * q : 0.f;
-                      }
-                  });
-    }
+    if (kStatic && S.sCount)
+        walk_runs(cx, cy, cz, S.Ms, S.magicMs, stable, S.sBegin, S.sBegin + S.sCount, [&](uint32_t s0, uint32_t e0) {
+            for (uint32_t j = s0; j < e0; ++j) dWall(j);
+        });
     const float rho = S.selfDensity + (sumDyn * S.mass + sumSt * S.mass) * S.C;
     // Only this lane writes pos[i].w; no lane of this pass reads any .w.
     reinterpret_cast<float*>(pos)[4ull * i + 3] = S.k * (rho - S.rho0);
@@ -325,13 +317,14 @@ __global__ __launch_bounds__(kBlock) void k_force_cw(const BlockDesc* __restrict
     const uint32_t i = wbegin + lane;
     const bool valid = lane < wcount;
     const uint32_t iv = valid ? i : wbegin;
-    const float4 xi = pos[iv];
-    const float4 vi = vel[iv];
+    const float4 xi = ld16(pos, iv);
+    const float4 vi = ld16(vel, iv);
     const float pi = xi.w;  // this step's pressure (density pass)
     const float h = S.h, h2 = S.h2;
     const uint32_t cx = cell_coord(xi.x, h), cy = cell_coord(xi.y, h), cz = cell_coord(xi.z, h);
     const uint32_t code = morton3(cx, cy, cz);
     const uint32_t* table = tab + S.tBegin;
+    const uint32_t* stable = stab + S.stBegin;
     const uint32_t lo = S.pBegin, hi = S.pBegin + S.pCount;
     uint64_t m[4];
     {
@@ -354,80 +347,80 @@ __global__ __launch_bounds__(kBlock) void k_force_cw(const BlockDesc* __restrict
     float zeroSp = 0.f;
     bool anyZero = false;
     // One dynamic candidate j (SF:557-589), with its position (pressure in w)
-    // and velocity already loaded.
+    // and velocity already loaded. Candidates out of range (or j == i) get zero
+    // kernel weights, so they add exact zeros.
     auto pair_at = [&](uint32_t j, const float4 xj, const float4 vj) {
         const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
         const float r2 = dx * dx + dy * dy + dz * dz;
         const bool ok = j != i && r2 <= h2;
         const float r = __builtin_amdgcn_sqrtf(r2);
-        const bool zero = r2 == 0.f || r == 0.f;
+        const bool zero = r == 0.f;  // r2 == 0 implies r == 0
         const float inv = zero ? 0.f : __builtin_amdgcn_rcpf(r);
         const float q = h - r;
-        const bool in = r < h;
+        const bool in = ok && r < h;
         const float w1 = in ? -3.0f * q * q : 0.f;
         const float w2 = in ? 6.0f * q : 0.f;
         const float sp = (pi + xj.w) * w1;
-        fpx += ok ? (dx * inv) * sp : 0.f;
-        fpy += ok ? (dy * inv) * sp : 0.f;
-        fpz += ok ? (dz * inv) * sp : 0.f;
+        fpx += (dx * inv) * sp;
+        fpy += (dy * inv) * sp;
+        fpz += (dz * inv) * sp;
         zeroSp += ok && zero ? sp : 0.f;
         anyZero |= ok && zero;
-        fvx += ok ? (vj.x - vi.x) * w2 : 0.f;
-        fvy += ok ? (vj.y - vi.y) * w2 : 0.f;
-        fvz += ok ? (vj.z - vi.z) * w2 : 0.f;
+        fvx += (vj.x - vi.x) * w2;
+        fvy += (vj.y - vi.y) * w2;
+        fvz += (vj.z - vi.z) * w2;
     };
-    auto pair = [&](uint32_t j) { pair_at(j, pos[j], vel[j]); };
     auto pair4 = [&](const uint32_t (&j)[4]) {
         float4 xj[4], vj[4];
 #pragma unroll
         for (uint32_t u = 0; u < 4; ++u) {
-            xj[u] = pos[j[u]];
-            vj[u] = vel[j[u]];
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) {
-            settle(xj[u]);
-            settle(vj[u]);
+            xj[u] = ld16(pos, j[u]);
+            vj[u] = ld16(vel, j[u]);
         }
 #pragma unroll
         for (uint32_t u = 0; u < 4; ++u) pair_at(j[u], xj[u], vj[u]);
+    };
+    // One static candidate (SF:590-623): |p_i| for both sides, viscosity scaled by 0.
+    auto wall = [&](uint32_t j) {
+        const float4 xj = ld16(spos, j);
+        const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 > h2) return;
+        const float r = __builtin_amdgcn_sqrtf(r2);
+        const bool zero = r == 0.f;
+        const float inv = zero ? 0.f : __builtin_amdgcn_rcpf(r);
+        const float q = h - r;
+        const bool in = r < h;
+        const float sp = absP * (in ? -3.0f * q * q : 0.f);
+        fsx += (dx * inv) * sp;
+        fsy += (dy * inv) * sp;
+        fsz += (dz * inv) * sp;
+        zeroSp += zero ? sp : 0.f;
+        anyZero |= zero;
+        sumD2s += in ? 6.0f * q : 0.f;
     };
     for_each_group(
         lane, wave, valid, code, cx, cy, cz, S, table, lo, hi, gt,
         [&](uint32_t, uint32_t, uint32_t, uint32_t, uint32_t) {},
         [&](uint32_t slot, uint32_t crowded) {
             walk_mask(m, gt.start[wave][slot], i, pair4);
-            // Crowded buckets: their particles past the kSlots-th.
+            // Crowded buckets: their particles past the kSlots-th, four at a time.
             for (; crowded; crowded &= crowded - 1u) {
                 const uint32_t bi = (uint32_t)__builtin_ctz(crowded);
                 const uint32_t s0 = gt.start[wave][slot][bi], n = gt.cnt[wave][slot][bi];
-                for (uint32_t t = kSlots; t < n; ++t) pair(s0 + t);
+                for (uint32_t t = kSlots; t < n; t += 4u) {
+                    uint32_t j[4];
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u) j[u] = t + u < n ? s0 + t + u : i;
+                    pair4(j);
+                }
             }
         });
     if (!valid) return;
-    if (kStatic && S.sCount) {
-        walk_runs(cx, cy, cz, S.Ms, S.magicMs, stab + S.stBegin, S.sBegin, S.sBegin + S.sCount,
-                  [&](uint32_t s, uint32_t e) {
-                      for (uint32_t j = s; j < e; ++j) {
-                          const float4 xj = spos[j];
-                          const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-                          const float r2 = dx * dx + dy * dy + dz * dz;
-                          if (r2 > h2) continue;
-                          const float r = __builtin_amdgcn_sqrtf(r2);
-                          const bool zero = r2 == 0.f || r == 0.f;
-                          const float inv = zero ? 0.f : __builtin_amdgcn_rcpf(r);
-                          const float q = h - r;
-                          const bool in = r < h;
-                          const float sp = absP * (in ? -3.0f * q * q : 0.f);
-                          fsx += (dx * inv) * sp;
-                          fsy += (dy * inv) * sp;
-                          fsz += (dz * inv) * sp;
-                          zeroSp += zero ? sp : 0.f;
-                          anyZero |= zero;
-                          sumD2s += in ? 6.0f * q : 0.f;
-                      }
-                  });
-    }
+    if (kStatic && S.sCount)
+        walk_runs(cx, cy, cz, S.Ms, S.magicMs, stable, S.sBegin, S.sBegin + S.sCount, [&](uint32_t s0, uint32_t e0) {
+            for (uint32_t j = s0; j < e0; ++j) wall(j);
+        });
     if (anyZero) {
         const float3 d = random_direction((float)(ids[i] - S.pBegin));
         fpx += d.x * zeroSp;

@@ -461,8 +461,8 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
     }
     uint64_t L = 0;
     for (uint32_t s = 0; s < n_scenes; ++s) L += 2 * (dyn_offsets[s + 1] - dyn_offsets[s]) + 1;
-    if (N >= (1ull << 31) || L >= (1ull << 32) - kScanTile || Ns >= (1ull << 32)) {
-        h->err = "sph_batch_initialize: particle counts exceed the 32-bit index range";
+    if (N >= (1ull << 28) || L >= (1ull << 32) - kScanTile || Ns >= (1ull << 28)) {
+        h->err = "sph_batch_initialize: more than 2^28 - 1 dynamic or static particles";
         return SPH_E_INVALID;
     }
     h->nScenes = n_scenes;
