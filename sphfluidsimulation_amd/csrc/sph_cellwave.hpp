@@ -138,11 +138,43 @@ __device__ inline RoundCand round_candidate(uint32_t r, uint32_t lane, uint32_t 
     return c;
 }
 
-// Iterates the groups of a wave in batches: phase1(slot, g0, g1, start27, cnt27)
-// runs wave-uniformly per group (lanes g0..g1-1 are its particles), then
-// phase2(slot, crowded) runs on the lanes of the batch's groups with `slot`
-// their group's table slot and `crowded` the bit set of its buckets holding more
-// than kSlots particles.
+// Table reads of one cell group: its lanes g0..g1-1 and, on lanes 0..26, the
+// raw table entries of neighbour cell `lane` in the reference's x-y-z visit
+// order (SF:397-404).
+struct GroupFetch {
+    uint32_t g0, g1, tb, te;
+};
+
+__device__ inline GroupFetch fetch_group(uint64_t& heads, uint64_t stops, uint32_t lane, uint32_t cx, uint32_t cy,
+                                         uint32_t cz, const SceneDev& S, const uint32_t* __restrict__ table,
+                                         uint32_t lo) {
+    GroupFetch f;
+    f.g0 = (uint32_t)__builtin_ctzll(heads);
+    heads &= heads - 1ull;
+    const uint64_t above = f.g0 < 63u ? (stops & ~((2ull << f.g0) - 1ull)) : 0ull;
+    f.g1 = above ? (uint32_t)__builtin_ctzll(above) : 64u;
+    const uint32_t gx = (uint32_t)__builtin_amdgcn_readlane((int)cx, (int)f.g0);
+    const uint32_t gy = (uint32_t)__builtin_amdgcn_readlane((int)cy, (int)f.g0);
+    const uint32_t gz = (uint32_t)__builtin_amdgcn_readlane((int)cz, (int)f.g0);
+    // Every lane reads (lanes past 26 repeat cell 26; the reader ignores them), so
+    // the reads are not in a branch and a wait for later loads need not cover them.
+    const uint32_t n = lane < 27u ? lane : 26u;
+    const uint32_t a = n / 9u, b = (n / 3u) % 3u, c = n % 3u;
+    const uint32_t bk = fastmod(morton3(gx + a - 1u, gy + b - 1u, gz + c - 1u), S.magicM, S.M);
+    f.tb = table[bk];
+    f.te = table[bk + 1];
+    return f;
+}
+
+// Iterates the groups of a wave in batches:
+// phase1(slot, g0, g1, start27, cnt27, between) runs wave-uniformly per group
+// (lanes g0..g1-1 are its particles; lanes 0..26 hold the clamped start and
+// count of its neighbour buckets) and calls between() once, after issuing its
+// own loads: between() issues the next group's table reads (unconditionally, so
+// the wait for the group's own loads never has to include them), so they
+// overlap the group's work. Then phase2(slot, crowded) runs on the lanes of the batch's
+// groups with `slot` their group's table slot and `crowded` the bit set of its
+// buckets holding more than kSlots particles.
 template <class P1, class P2>
 __device__ inline void for_each_group(uint32_t lane, uint32_t wave, bool valid, uint32_t code, uint32_t cx,
                                       uint32_t cy, uint32_t cz, const SceneDev& S, const uint32_t* __restrict__ table,
@@ -152,35 +184,27 @@ __device__ inline void for_each_group(uint32_t lane, uint32_t wave, bool valid, 
     const uint64_t stops = heads | ~ballot64(valid);  // a group ends at the next head or invalid lane
     while (heads) {
         uint32_t mySlot = 0xFFFFFFFFu, myCrowded = 0u;
-        uint32_t slot = 0;
-        for (; heads && slot < kGroupBatch; ++slot) {
-            const uint32_t g0 = (uint32_t)__builtin_ctzll(heads);
-            heads &= heads - 1ull;
-            const uint64_t above = g0 < 63u ? (stops & ~((2ull << g0) - 1ull)) : 0ull;
-            const uint32_t g1 = above ? (uint32_t)__builtin_ctzll(above) : 64u;
-            const uint32_t gx = (uint32_t)__builtin_amdgcn_readlane((int)cx, (int)g0);
-            const uint32_t gy = (uint32_t)__builtin_amdgcn_readlane((int)cy, (int)g0);
-            const uint32_t gz = (uint32_t)__builtin_amdgcn_readlane((int)cz, (int)g0);
-            // Lanes 0..26: neighbour cell `lane` in the reference's x-y-z visit
-            // order (SF:397-404) and its bucket ranges.
-            uint32_t start27 = lo, cnt27 = 0u;
+        GroupFetch cur = fetch_group(heads, stops, lane, cx, cy, cz, S, table, lo);
+        for (uint32_t slot = 0;; ++slot) {
+            const uint32_t s0 = cur.tb > lo ? cur.tb : lo, e0 = cur.te < hi ? cur.te : hi;
+            const uint32_t start27 = s0, cnt27 = lane < 27u && e0 > s0 ? e0 - s0 : 0u;  // empty past lane 26
             if (lane < 27u) {
-                const uint32_t a = lane / 9u, b = (lane / 3u) % 3u, c = lane % 3u;
-                const uint32_t ncode = morton3(gx + a - 1u, gy + b - 1u, gz + c - 1u);
-                const uint32_t bk = fastmod(ncode, S.magicM, S.M);
-                const uint32_t tb = table[bk], te = table[bk + 1];
-                const uint32_t s0 = tb > lo ? tb : lo, e0 = te < hi ? te : hi;
-                start27 = s0;
-                cnt27 = e0 > s0 ? e0 - s0 : 0u;
                 gt.start[wave][slot][lane] = start27;
                 gt.cnt[wave][slot][lane] = cnt27;
             }
             const uint32_t crowded = (uint32_t)ballot64(cnt27 > kSlots);  // buckets 0..26
-            if (lane >= g0 && lane < g1) {
+            if (lane >= cur.g0 && lane < cur.g1) {
                 mySlot = slot;
                 myCrowded = crowded;
             }
-            phase1(slot, g0, g1, start27, cnt27);
+            const bool more = heads != 0ull && slot + 1u < kGroupBatch;
+            GroupFetch nxt;
+            uint64_t rest = more ? heads : 1ull;  // at the batch end: a harmless read of group 0's table
+            phase1(slot, cur.g0, cur.g1, start27, cnt27,
+                   [&]() { nxt = fetch_group(rest, stops, lane, cx, cy, cz, S, table, lo); });
+            if (!more) break;
+            heads = rest;
+            cur = nxt;
         }
         __builtin_amdgcn_wave_barrier();
         if (mySlot != 0xFFFFFFFFu) phase2(mySlot, myCrowded);
@@ -244,7 +268,7 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
     };
     for_each_group(
         lane, wave, valid, code, cx, cy, cz, S, table, lo, hi, gt,
-        [&](uint32_t, uint32_t g0, uint32_t g1, uint32_t start27, uint32_t cnt27) {
+        [&](uint32_t, uint32_t g0, uint32_t g1, uint32_t start27, uint32_t cnt27, auto&& between) {
             // The particle itself passes its own test (r = 0); d4 skips it. Empty
             // slots are masked out with the (wave-uniform) slot masks.
             float4 xc[4];
@@ -255,6 +279,7 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
                 xc[r] = ld16(pos, c.valid ? c.j : lo);
                 occupied[r] = ballot64(c.valid);
             }
+            between();
             for (uint32_t t = g0; t < g1; ++t) {
                 const float px = readlanef(xi.x, t), py = readlanef(xi.y, t), pz = readlanef(xi.z, t);
                 uint64_t b[4];
@@ -401,7 +426,7 @@ __global__ __launch_bounds__(kBlock) void k_force_cw(const BlockDesc* __restrict
     };
     for_each_group(
         lane, wave, valid, code, cx, cy, cz, S, table, lo, hi, gt,
-        [&](uint32_t, uint32_t, uint32_t, uint32_t, uint32_t) {},
+        [&](uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, auto&& between) { between(); },
         [&](uint32_t slot, uint32_t crowded) {
             walk_mask(m, gt.start[wave][slot], i, pair4);
             // Crowded buckets: their particles past the kSlots-th, four at a time.
