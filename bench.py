@@ -110,6 +110,8 @@ def main():
     ap.add_argument("--walk", default="cellwave", choices=["cellwave", "tile", "runs", "cells"],
                     help="neighbour passes: cell-group waves (default), LDS-staged bucket tiles, "
                          "HBM Morton runs, or cell by cell")
+    ap.add_argument("--autotune", action="store_true",
+                    help="after initialize, let the engine time every walk and keep the fastest")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--profile-all", action="store_true", help="time every stage with events (default: dominant)")
@@ -155,6 +157,10 @@ def main():
         e = SimulationEngineHIP(0)
         e.Initialize(scene_list if len(scene_list) > 1 else scene_list[0], flags=flags)
         engines.append(e)
+    if args.autotune:
+        chosen = {e.autotune(dt, 4) for e in engines}
+        args.walk = "/".join(sorted(chosen))
+        log(f"[bench] autotune chose {args.walk}")
     n_total = sum(e.n for e in engines)
     M_total = sum(e.hash_map_size(s) for e in engines for s in range(e.n_scenes))
     Ms_total = sum(sc.n_static for sc in scene_list)
@@ -223,6 +229,7 @@ def main():
             "dt": dt,
             "hip_graph": bool(args.graph),
             "walk": args.walk,
+            "autotune": bool(args.autotune),
         },
         "roofline": {
             "bound": "hbm",

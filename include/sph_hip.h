@@ -197,6 +197,19 @@ uint64_t sph_hash_map_size(sph_handle h, uint32_t scene);
 int sph_set_profiling(sph_handle h, uint32_t stage_mask);
 int sph_stage_times(sph_handle h, double* ms, uint64_t* launches);
 
+/* Autotuner with no counterpart in the reference (whose engines have one fixed
+ * neighbour loop, SPHFunctions.cpp:397-404): times every neighbour-pass variant
+ * (cell-group waves, Morton runs, LDS bucket tiles) for 2 warm-up steps plus
+ * trial_steps (rounded up to an even count, at least 2) steps of dt on the
+ * current state, keeps the fastest for later updates and writes its walk flag
+ * (0, SPH_FLAG_RUN_WALK or SPH_FLAG_TILE_WALK) to *chosen. The state, the
+ * simulation time and the results of later updates (within the float tolerance
+ * of the variants' summation orders) are as if it had not been called: each
+ * trial runs from a device copy of the state. Synchronous; allocates scratch
+ * copies of positions, velocities, ids and the cell-start table. An engine
+ * initialized with SPH_FLAG_CELL_WALK keeps that walk. */
+int sph_autotune(sph_handle h, float dt, uint32_t trial_steps, uint32_t* chosen);
+
 /* Measurement aid with no counterpart in the reference: launches `repeats` plain
  * copies of `bytes` bytes between two device buffers (kernel k_copy_probe), so
  * that HBM byte counters read under a profiler can be scaled against a kernel

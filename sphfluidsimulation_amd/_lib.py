@@ -88,6 +88,7 @@ SIGNATURES = {
     "sph_set_profiling": (ctypes.c_int, [_P, ctypes.c_uint32]),
     "sph_stage_times": (ctypes.c_int, [_P, _F64P, _U64P]),
     "sph_copy_probe": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint32]),
+    "sph_autotune": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_uint32, _U32P]),
 }
 
 _lib = None

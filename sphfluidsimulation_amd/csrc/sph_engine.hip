@@ -939,6 +939,109 @@ int sph_stage_times(sph_handle h, double* ms, uint64_t* launches) {
     return SPH_OK;
 }
 
+int sph_autotune(sph_handle h, float dt, uint32_t trial_steps, uint32_t* chosen) {
+    if (!valid(h)) return SPH_E_INVALID;
+    if (!h->initialized) {
+        h->err = "sph_autotune: engine not initialized";
+        return SPH_E_STATE;
+    }
+    const uint32_t walkBits = SPH_FLAG_CELL_WALK | SPH_FLAG_RUN_WALK | SPH_FLAG_TILE_WALK;
+    if (h->N == 0 || !h->mask) {  // nothing to time, or the engine was set up for the cell walk only
+        if (chosen) *chosen = h->flags & walkBits;
+        return SPH_OK;
+    }
+    SPH_HIP(h, hipSetDevice(h->device));
+    const uint32_t warm = 2, timed = std::max<uint32_t>(2u, (trial_steps + 1u) & ~1u);  // even: same parity after
+    const uint64_t N = h->N, L = h->L;
+    const int c = h->cur;
+    const uint32_t savedProfiling = h->profiling;
+    h->profiling = 0;
+    float4 *bPos = nullptr, *bVel = nullptr;
+    uint32_t *bIds = nullptr, *bTab = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = SPH_OK;
+    auto finish = [&](int code) {
+        dfree(bPos);
+        dfree(bVel);
+        dfree(bIds);
+        dfree(bTab);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        h->profiling = savedProfiling;
+        return code;
+    };
+    // Snapshot of the state a step starts from: tab[c] holds the cell starts,
+    // tab[c ^ 1] is all zero, and the scan's status arrays are in the state
+    // parity c expects after any even number of steps.
+    if ((rc = dalloc(h, &bPos, N)) || (rc = dalloc(h, &bVel, N)) || (rc = dalloc(h, &bIds, N)) ||
+        (rc = dalloc(h, &bTab, L)))
+        return finish(rc);
+    auto copy = [&](void* dst, const void* src, uint64_t bytes) {
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, h->stream);
+    };
+    auto restore = [&]() -> hipError_t {
+        hipError_t e;
+        if ((e = copy(h->pos, bPos, N * sizeof(float4))) != hipSuccess) return e;
+        if ((e = copy(h->vel, bVel, N * sizeof(float4))) != hipSuccess) return e;
+        if ((e = copy(h->ids[c], bIds, N * sizeof(uint32_t))) != hipSuccess) return e;
+        if ((e = copy(h->tab[c], bTab, L * sizeof(uint32_t))) != hipSuccess) return e;
+        return hipMemsetAsync(h->tab[c ^ 1], 0, L * sizeof(uint32_t), h->stream);
+    };
+    hipError_t e = copy(bPos, h->pos, N * sizeof(float4));
+    if (e == hipSuccess) e = copy(bVel, h->vel, N * sizeof(float4));
+    if (e == hipSuccess) e = copy(bIds, h->ids[c], N * sizeof(uint32_t));
+    if (e == hipSuccess) e = copy(bTab, h->tab[c], L * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e != hipSuccess) {
+        h->err = std::string("sph_autotune: ") + hipGetErrorString(e);
+        return finish(SPH_E_HIP);
+    }
+    const uint32_t candidates[3] = {0u, SPH_FLAG_RUN_WALK, SPH_FLAG_TILE_WALK};
+    const uint32_t base = h->flags & ~walkBits;
+    uint32_t best = h->flags & walkBits;
+    float bestMs = 0.f;
+    bool first = true;
+    for (uint32_t w : candidates) {
+        h->flags = base | w;
+        for (uint32_t s = 0; s < warm + timed; ++s) {
+            if (s == warm && (e = hipEventRecord(e0, h->stream)) != hipSuccess) break;
+            if ((rc = enqueue_step(h, dt, false))) return finish(rc);
+            h->cur ^= 1;
+        }
+        if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        h->cur = c;
+        if (e == hipSuccess) e = restore();
+        if (e != hipSuccess) {
+            h->flags = base | best;
+            h->err = std::string("sph_autotune: ") + hipGetErrorString(e);
+            return finish(SPH_E_HIP);
+        }
+        if (first || ms < bestMs) {
+            best = w;
+            bestMs = ms;
+            first = false;
+        }
+    }
+    h->flags = base | best;
+    for (int p = 0; p < 2; ++p)  // captured steps may hold another walk's kernels
+        if (h->graph[p]) {
+            (void)hipGraphExecDestroy(h->graph[p]);
+            h->graph[p] = nullptr;
+        }
+    h->debugValid = false;
+    e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) {
+        h->err = std::string("sph_autotune: ") + hipGetErrorString(e);
+        return finish(SPH_E_HIP);
+    }
+    if (chosen) *chosen = best;
+    return finish(SPH_OK);
+}
+
 int sph_copy_probe(sph_handle h, uint64_t bytes, uint32_t repeats) {
     if (!valid(h) || bytes < 16) return SPH_E_INVALID;
     SPH_HIP(h, hipSetDevice(h->device));

@@ -202,6 +202,16 @@ class SimulationEngineHIP:
         self._check(self._lib.sph_stage_times(self._h, _ptr(ms, ctypes.c_double), _ptr(n, ctypes.c_uint64)))
         return {name: (float(ms[i]), int(n[i])) for i, name in enumerate(_lib.STAGES)}
 
+    WALK_NAMES = {0: "cellwave", _lib.SPH_FLAG_RUN_WALK: "runs", _lib.SPH_FLAG_TILE_WALK: "tile",
+                  _lib.SPH_FLAG_CELL_WALK: "cells"}
+
+    def autotune(self, dt: float, trial_steps: int = 4) -> str:
+        """Time every neighbour-pass variant on the current state, keep the
+        fastest; the state is left as it was. Returns the chosen walk's name."""
+        chosen = ctypes.c_uint32(0)
+        self._check(self._lib.sph_autotune(self._h, ctypes.c_float(dt), int(trial_steps), ctypes.byref(chosen)))
+        return self.WALK_NAMES[int(chosen.value)]
+
     def copy_probe(self, nbytes: int, repeats: int = 1):
         """Plain device copy of nbytes (counter calibration under a profiler)."""
         self._check(self._lib.sph_copy_probe(self._h, int(nbytes), int(repeats)))

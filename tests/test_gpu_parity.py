@@ -299,6 +299,40 @@ def test_cell_occupancy_extremes(kind, walk):
     _check_step(sc, got, ref, 1e-3)
 
 
+def test_autotune_leaves_state_unchanged(s16k):
+    say("autotune: every walk timed from a copy of the state")
+    e = SimulationEngineHIP(0)
+    e.Initialize(s16k)
+    e.Update(1e-3, 3)
+    pos0, vel0, p0 = e.read()
+    keys0 = e.read_keys()
+    starts0 = e.read_cell_starts()[0]
+    t0 = e.GetSimulationTime()
+    walk = e.autotune(1e-3, 4)
+    say(f"autotune chose {walk}")
+    assert walk in ("cellwave", "runs", "tile")
+    pos1, vel1, p1 = e.read()
+    np.testing.assert_array_equal(pos1, pos0)
+    np.testing.assert_array_equal(vel1, vel0)
+    np.testing.assert_array_equal(p1, p0)
+    np.testing.assert_array_equal(e.read_keys(), keys0)
+    np.testing.assert_array_equal(e.read_cell_starts()[0], starts0)
+    assert e.GetSimulationTime() == t0
+    assert e.validate() == 0
+    # Later steps match an engine that never ran the trials.
+    ref = SimulationEngineHIP(0)
+    ref.Initialize(s16k)
+    ref.Update(1e-3, 8)
+    e.Update(1e-3, 5)
+    pa, va, _ = e.read()
+    pb, vb, _ = ref.read()
+    assert float(np.max(np.abs(pa - pb))) < 1e-5
+    assert float(np.max(np.abs(va - vb))) < 1e-3
+    assert e.validate() == 0
+    e.close()
+    ref.close()
+
+
 def test_update_before_initialize_and_empty():
     say("error conventions")
     e = SimulationEngineHIP(0)
