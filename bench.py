@@ -61,10 +61,19 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def load_traffic(workload: str):
-    """HBM bytes per launch of the dominant kernel from a committed rocprofv3
-    --pmc summary (profiles/<kernel>_<workload>_pmc.md, line 'traffic_bytes: X'), if any."""
-    path = os.path.join(ROOT, "profiles", f"k_force_{workload}_pmc.md")
+# Kernel of each timed stage, per neighbour-pass variant.
+KERNELS = {
+    "cellwave": {"density": "k_density_cw", "force": "k_force_cw"},
+    "runs": {"density": "k_density_runs", "force": "k_force_runs"},
+    "tile": {"density": "k_density_tile", "force": "k_force_tile"},
+    "cells": {"density": "k_density", "force": "k_force"},
+}
+
+
+def load_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of a kernel from a committed rocprofv3 --pmc summary
+    (profiles/<kernel>_<workload>_pmc.md, line 'traffic_bytes: X'), if any."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_pmc.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -205,7 +214,9 @@ def main():
     avg_s = timed[dom][0] / 1e3 / (timed[dom][1] / len(engines))
     achieved = sb[dom] / avg_s / 1e9
     workload = scene_list[0].name if len(scene_list) == 1 else f"{args.config}-{len(scene_list)}-scenes"
-    traffic = load_traffic(workload) if dom == "force" else None
+    walk_kernels = KERNELS.get(args.walk, {})
+    kernel = walk_kernels.get(dom, "k_" + dom)
+    traffic = load_traffic(kernel, workload)
     b_step = step_bytes(n_total, M_total, Ms_total)
     result = {
         "metric": METRIC,
@@ -233,7 +244,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_" + dom,
+            "kernel": kernel,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
