@@ -119,6 +119,8 @@ def main():
     ap.add_argument("--walk", default="cellwave", choices=["cellwave", "tile", "runs", "cells"],
                     help="neighbour passes: cell-group waves (default), LDS-staged bucket tiles, "
                          "HBM Morton runs, or cell by cell")
+    ap.add_argument("--mixed", action="store_true",
+                    help="mixed precision: FP16-offset candidate filter, Kahan-compensated neighbour sums")
     ap.add_argument("--autotune", action="store_true",
                     help="after initialize, let the engine time every walk and keep the fastest")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -135,8 +137,8 @@ def main():
 
 
     from sphfluidsimulation_amd import SimulationEngineHIP, scenes
-    from sphfluidsimulation_amd._lib import (SPH_FLAG_CELL_WALK, SPH_FLAG_HIP_GRAPH, SPH_FLAG_RUN_WALK,
-                                             SPH_FLAG_TILE_WALK)
+    from sphfluidsimulation_amd._lib import (SPH_FLAG_CELL_WALK, SPH_FLAG_FP16_OFFSETS, SPH_FLAG_HIP_GRAPH,
+                                             SPH_FLAG_RUN_WALK, SPH_FLAG_TILE_WALK)
 
     if args.copy_probe:
         e = SimulationEngineHIP(0)
@@ -154,7 +156,8 @@ def main():
         f"{sum(s.n for s in scene_list)} particles, generated in {time.perf_counter() - t0:.1f} s")
 
     flags = ((SPH_FLAG_HIP_GRAPH if args.graph else 0) | (SPH_FLAG_CELL_WALK if args.walk == "cells" else 0)
-             | (SPH_FLAG_RUN_WALK if args.walk == "runs" else 0) | (SPH_FLAG_TILE_WALK if args.walk == "tile" else 0))
+             | (SPH_FLAG_RUN_WALK if args.walk == "runs" else 0) | (SPH_FLAG_TILE_WALK if args.walk == "tile" else 0)
+             | (SPH_FLAG_FP16_OFFSETS if args.mixed else 0))
     plain = args.config.endswith("-plain")
     engines = []
     if plain:
@@ -241,6 +244,7 @@ def main():
             "hip_graph": bool(args.graph),
             "walk": args.walk,
             "autotune": bool(args.autotune),
+            "precision": "mixed (fp16 filter, kahan sums)" if args.mixed else "fp32",
         },
         "roofline": {
             "bound": "hbm",
@@ -259,6 +263,8 @@ def main():
             "frac": b_step * steps / elapsed / 1e9 / HBM_PEAK_GBS,
         },
         "stage_ms_per_step": {k: v[0] / max(1, v[1] / len(engines)) for k, v in timed.items()},
+        # Achieved GB/s of each timed stage: its algorithmic bytes per launch over its average launch time.
+        "stage_GBs": {k: sb[k] / (v[0] / 1e3 / max(1, v[1] / len(engines))) / 1e9 for k, v in timed.items()},
     }
     for e in engines:
         e.close()

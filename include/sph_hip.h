@@ -41,7 +41,10 @@ extern "C" {
 /* sph_initialize / sph_batch_initialize flags. */
 #define SPH_FLAG_HIP_GRAPH 0x1u     /* replay each step from a captured hipGraph */
 #define SPH_FLAG_RECORD_DEBUG 0x2u  /* keep per-particle density and acceleration of the last step */
-#define SPH_FLAG_FP16_OFFSETS 0x4u  /* mixed precision: FP16 cell-centre offsets, Kahan density sums */
+#define SPH_FLAG_FP16_OFFSETS 0x4u  /* mixed precision (default walk only): the density pass's candidate
+                                       filter tests FP16 cell offsets (a proven superset of the h-ball),
+                                       every neighbour sum is Kahan-compensated FP32; kernel values
+                                       stay FP32, conserved reductions FP64 */
 #define SPH_FLAG_CELL_WALK 0x8u     /* neighbour passes read every candidate from HBM, one cell at a
                                        time: the reference's loop shape, kept for comparison */
 #define SPH_FLAG_RUN_WALK 0x10u     /* neighbour passes read candidates from HBM in merged Morton runs */

@@ -41,13 +41,18 @@ struct GroupTables {
 
 __device__ inline uint64_t ballot64(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
-// Lane t of m[r] takes the wave-uniform ballot b[r], r = 0..3 (v_writelane_b32
+// A neighbour mask as it is built: 4 rounds x 64 bits held as eight 32-bit halves
+// (round r in w[2r] low, w[2r + 1] high), so that writing a lane never has to
+// re-pair 64-bit registers.
+struct Mask8 {
+    uint32_t w[8];
+};
+
+// Lane t of round r takes the wave-uniform ballot b[r], r = 0..3 (v_writelane_b32
 // with the lane select in m0, since the data operand already uses the one scalar
 // operand a VALU instruction may read; the s_nop covers the wait states between
 // writing m0 and reading it as a lane select).
-__device__ inline void writelane4(uint64_t (&m)[4], const uint64_t (&b)[4], uint32_t t) {
-    uint32_t m0 = (uint32_t)m[0], m1 = (uint32_t)(m[0] >> 32), m2 = (uint32_t)m[1], m3 = (uint32_t)(m[1] >> 32);
-    uint32_t m4 = (uint32_t)m[2], m5 = (uint32_t)(m[2] >> 32), m6 = (uint32_t)m[3], m7 = (uint32_t)(m[3] >> 32);
+__device__ inline void writelane4(Mask8& m, const uint64_t (&b)[4], uint32_t t) {
     asm volatile(
         "s_nop 4\n\t"
         "v_writelane_b32 %0, %8, m0\n\t"
@@ -58,14 +63,11 @@ __device__ inline void writelane4(uint64_t (&m)[4], const uint64_t (&b)[4], uint
         "v_writelane_b32 %5, %13, m0\n\t"
         "v_writelane_b32 %6, %14, m0\n\t"
         "v_writelane_b32 %7, %15, m0"
-        : "+v"(m0), "+v"(m1), "+v"(m2), "+v"(m3), "+v"(m4), "+v"(m5), "+v"(m6), "+v"(m7)
+        : "+v"(m.w[0]), "+v"(m.w[1]), "+v"(m.w[2]), "+v"(m.w[3]), "+v"(m.w[4]), "+v"(m.w[5]), "+v"(m.w[6]),
+          "+v"(m.w[7])
         : "s"((uint32_t)b[0]), "s"((uint32_t)(b[0] >> 32)), "s"((uint32_t)b[1]), "s"((uint32_t)(b[1] >> 32)),
           "s"((uint32_t)b[2]), "s"((uint32_t)(b[2] >> 32)), "s"((uint32_t)b[3]), "s"((uint32_t)(b[3] >> 32)),
           "{m0}"(t));
-    m[0] = ((uint64_t)m1 << 32) | m0;
-    m[1] = ((uint64_t)m3 << 32) | m2;
-    m[2] = ((uint64_t)m5 << 32) | m4;
-    m[3] = ((uint64_t)m7 << 32) | m6;
 }
 
 // Record j of an array of 16-byte records through a 32-bit byte offset (the
@@ -120,6 +122,41 @@ __device__ inline void walk_mask(const uint64_t (&m)[4], const uint32_t* start, 
 __device__ inline float readlanef(float v, uint32_t lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)lane));
 }
+
+// A neighbour sum: plain FP32, or Kahan-compensated FP32 (mixed-precision mode).
+template <bool kKahan>
+struct Acc {
+    float s = 0.f, c = 0.f;
+    __device__ void add(float v) {
+        if (kKahan) {
+            const float y = v - c;
+            const float t = s + y;
+            c = (t - s) - y;
+            s = t;
+        } else {
+            s += v;
+        }
+    }
+};
+
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+
+__device__ inline half2v pack_h2(float lo, float hi) { return half2v{(_Float16)lo, (_Float16)hi}; }
+__device__ inline uint32_t h2_bits(half2v v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ inline half2v h2_from(uint32_t b) { return __builtin_bit_cast(half2v, b); }
+
+// Mixed-precision candidate filter. Offsets are u = (x - c) / h from the corner c
+// of the group's cell, rounded to FP16 (corner and target share the formula, so
+// x - c is exact and the corner's own rounding cancels in differences). A target
+// in that cell has |u_t| <= 1 (a hair above with rounding) and a true neighbour
+// |u_c| < 2 (a hair above), so the rounded offsets are within 2^-11 and 2^-10 of
+// their FP32 values and the FP16 difference adds at most 2^-11: 2^-9 per axis in
+// all. For a pair with r <= h the computed |u_c - u_t|^2 is then at most
+// 1 + 2 sqrt(3) 2^-9 + 3 2^-18 plus five FP16 roundings (<= 1.7e-3): below
+// 1.0085. The test against kFilterR2 = 1.02 (1.0195 in FP16) keeps every
+// neighbour. Candidates far from the cell get large or infinite offsets and fail
+// it, as they should; the exact FP32 test follows in the mask walk.
+constexpr float kFilterR2 = 1.02f;
 
 // Candidate of round r on this lane: bucket 8r + lane/8, slot lane%8.
 struct RoundCand {
@@ -212,7 +249,7 @@ __device__ inline void for_each_group(uint32_t lane, uint32_t wave, bool valid, 
     }
 }
 
-template <bool kStatic, bool kDebug>
+template <bool kStatic, bool kDebug, bool kMixed>
 __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
                                                        const SceneDev* __restrict__ scenes, float4* pos,
                                                        const uint32_t* __restrict__ tab,
@@ -239,8 +276,19 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
     const uint32_t* stable = stab + S.stBegin;
     const uint32_t lo = S.pBegin, hi = S.pBegin + S.pCount;
 
-    float sumDyn = 0.f, sumSt = 0.f;
-    uint64_t m[4] = {0ull, 0ull, 0ull, 0ull};  // this lane's neighbour mask, rounds 0..3
+    Acc<kMixed> sumDyn, sumSt;
+    Mask8 m8 = {{0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u}};  // this lane's neighbour mask, rounds 0..3
+    // Mixed precision: this lane's cell corner and its FP16 offset from it, each
+    // coordinate twice (both halves), for the candidate filter.
+    const float invh = 1.0f / h;
+    uint32_t tux = 0u, tuy = 0u, tuz = 0u;
+    if (kMixed) {
+        const float ccx = (float)(int32_t)cx * h, ccy = (float)(int32_t)cy * h, ccz = (float)(int32_t)cz * h;
+        const float ux = (xi.x - ccx) * invh, uy = (xi.y - ccy) * invh, uz = (xi.z - ccz) * invh;
+        tux = h2_bits(pack_h2(ux, ux));
+        tuy = h2_bits(pack_h2(uy, uy));
+        tuz = h2_bits(pack_h2(uz, uz));
+    }
     // Exact contributions of four candidates j (SF:426-453).
     auto d4 = [&](const uint32_t (&j)[4]) {
         float4 xj[4];
@@ -252,7 +300,7 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
             const float r2 = dx * dx + dy * dy + dz * dz;
             const float r = __builtin_amdgcn_sqrtf(r2);
             const float q = h - r;
-            sumDyn += (r2 <= h2 && j[u] != i && r < h) ? q * q * q : 0.f;
+            sumDyn.add((r2 <= h2 && j[u] != i && r < h) ? q * q * q : 0.f);
         }
     };
     // Static neighbours (SF:455-496).
@@ -263,7 +311,7 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
         if (r2 <= h2) {
             const float r = __builtin_amdgcn_sqrtf(r2);
             const float q = h - r;
-            sumSt += (r < h) ? q * q * q : 0.f;
+            sumSt.add((r < h) ? q * q * q : 0.f);
         }
     };
     for_each_group(
@@ -280,6 +328,42 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
                 occupied[r] = ballot64(c.valid);
             }
             between();
+            if (kMixed) {
+                // FP16 filter, unless the group's particles span aliased cells (same
+                // Morton code, cells 1024 apart), which the FP32 test below handles.
+                const uint32_t gx = (uint32_t)__builtin_amdgcn_readlane((int)cx, (int)g0);
+                const uint32_t gy = (uint32_t)__builtin_amdgcn_readlane((int)cy, (int)g0);
+                const uint32_t gz = (uint32_t)__builtin_amdgcn_readlane((int)cz, (int)g0);
+                const bool inGroup = lane >= g0 && lane < g1;
+                if (ballot64(inGroup && (cx != gx || cy != gy || cz != gz)) == 0ull) {
+                    // The same corner formula as the lanes' own offsets above.
+                    const float gcx = (float)(int32_t)gx * h, gcy = (float)(int32_t)gy * h,
+                                gcz = (float)(int32_t)gz * h;
+                    half2v X[2], Y[2], Z[2];  // rounds (0, 1) and (2, 3)
+#pragma unroll
+                    for (uint32_t p = 0; p < 2; ++p) {
+                        X[p] = pack_h2((xc[2 * p].x - gcx) * invh, (xc[2 * p + 1].x - gcx) * invh);
+                        Y[p] = pack_h2((xc[2 * p].y - gcy) * invh, (xc[2 * p + 1].y - gcy) * invh);
+                        Z[p] = pack_h2((xc[2 * p].z - gcz) * invh, (xc[2 * p + 1].z - gcz) * invh);
+                    }
+                    const _Float16 R = (_Float16)kFilterR2;
+                    for (uint32_t t = g0; t < g1; ++t) {
+                        const half2v TX = h2_from((uint32_t)__builtin_amdgcn_readlane((int)tux, (int)t));
+                        const half2v TY = h2_from((uint32_t)__builtin_amdgcn_readlane((int)tuy, (int)t));
+                        const half2v TZ = h2_from((uint32_t)__builtin_amdgcn_readlane((int)tuz, (int)t));
+                        uint64_t b[4];
+#pragma unroll
+                        for (uint32_t p = 0; p < 2; ++p) {
+                            const half2v dx = X[p] - TX, dy = Y[p] - TY, dz = Z[p] - TZ;
+                            const half2v d2 = dx * dx + dy * dy + dz * dz;
+                            b[2 * p] = ballot64(d2.x <= R) & occupied[2 * p];
+                            b[2 * p + 1] = ballot64(d2.y <= R) & occupied[2 * p + 1];
+                        }
+                        writelane4(m8, b, t);
+                    }
+                    return;
+                }
+            }
             for (uint32_t t = g0; t < g1; ++t) {
                 const float px = readlanef(xi.x, t), py = readlanef(xi.y, t), pz = readlanef(xi.z, t);
                 uint64_t b[4];
@@ -288,10 +372,13 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
                     const float dx = xc[r].x - px, dy = xc[r].y - py, dz = xc[r].z - pz;
                     b[r] = ballot64(dx * dx + dy * dy + dz * dz <= h2wide) & occupied[r];
                 }
-                writelane4(m, b, t);
+                writelane4(m8, b, t);
             }
         },
         [&](uint32_t slot, uint32_t crowded) {
+            uint64_t m[4];
+#pragma unroll
+            for (uint32_t r = 0; r < 4; ++r) m[r] = ((uint64_t)m8.w[2 * r + 1] << 32) | m8.w[2 * r];
             walk_mask(m, gt.start[wave][slot], i, d4);
             // Crowded buckets: their particles past the kSlots-th, four at a time.
             for (; crowded; crowded &= crowded - 1u) {
@@ -310,16 +397,16 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
         walk_runs(cx, cy, cz, S.Ms, S.magicMs, stable, S.sBegin, S.sBegin + S.sCount, [&](uint32_t s0, uint32_t e0) {
             for (uint32_t j = s0; j < e0; ++j) dWall(j);
         });
-    const float rho = S.selfDensity + (sumDyn * S.mass + sumSt * S.mass) * S.C;
+    const float rho = S.selfDensity + (sumDyn.s * S.mass + sumSt.s * S.mass) * S.C;
     // Only this lane writes pos[i].w; no lane of this pass reads any .w.
     reinterpret_cast<float*>(pos)[4ull * i + 3] = S.k * (rho - S.rho0);
     if (kDebug) dens[i] = rho;
     uint4* mrow = reinterpret_cast<uint4*>(mask + (size_t)i * kCwMaskWords);
-    mrow[0] = make_uint4((uint32_t)m[0], (uint32_t)(m[0] >> 32), (uint32_t)m[1], (uint32_t)(m[1] >> 32));
-    mrow[1] = make_uint4((uint32_t)m[2], (uint32_t)(m[2] >> 32), (uint32_t)m[3], (uint32_t)(m[3] >> 32));
+    mrow[0] = make_uint4(m8.w[0], m8.w[1], m8.w[2], m8.w[3]);
+    mrow[1] = make_uint4(m8.w[4], m8.w[5], m8.w[6], m8.w[7]);
 }
 
-template <bool kStatic, bool kDebug, bool kTri>
+template <bool kStatic, bool kDebug, bool kTri, bool kMixed>
 __global__ __launch_bounds__(kBlock) void k_force_cw(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
                                                      const SceneDev* __restrict__ scenes,
                                                      const float4* __restrict__ pos, const float4* __restrict__ vel,
@@ -362,9 +449,7 @@ __global__ __launch_bounds__(kBlock) void k_force_cw(const BlockDesc* __restrict
     }
     const float rho = pi / S.k + S.rho0;  // SF:544
     const float absP = fabsf(pi);
-    float fpx = 0.f, fpy = 0.f, fpz = 0.f;
-    float fvx = 0.f, fvy = 0.f, fvz = 0.f;
-    float fsx = 0.f, fsy = 0.f, fsz = 0.f;
+    Acc<kMixed> fpx, fpy, fpz, fvx, fvy, fvz, fsx, fsy, fsz;
     float sumD2s = 0.f;
     // Pressure weights of coincident pairs (r == 0): their direction is the
     // particle's random direction (SF:566-570), the same for every such pair, so
@@ -386,14 +471,14 @@ __global__ __launch_bounds__(kBlock) void k_force_cw(const BlockDesc* __restrict
         const float w1 = in ? -3.0f * q * q : 0.f;
         const float w2 = in ? 6.0f * q : 0.f;
         const float sp = (pi + xj.w) * w1;
-        fpx += (dx * inv) * sp;
-        fpy += (dy * inv) * sp;
-        fpz += (dz * inv) * sp;
+        fpx.add((dx * inv) * sp);
+        fpy.add((dy * inv) * sp);
+        fpz.add((dz * inv) * sp);
         zeroSp += ok && zero ? sp : 0.f;
         anyZero |= ok && zero;
-        fvx += (vj.x - vi.x) * w2;
-        fvy += (vj.y - vi.y) * w2;
-        fvz += (vj.z - vi.z) * w2;
+        fvx.add((vj.x - vi.x) * w2);
+        fvy.add((vj.y - vi.y) * w2);
+        fvz.add((vj.z - vi.z) * w2);
     };
     auto pair4 = [&](const uint32_t (&j)[4]) {
         float4 xj[4], vj[4];
@@ -417,9 +502,9 @@ __global__ __launch_bounds__(kBlock) void k_force_cw(const BlockDesc* __restrict
         const float q = h - r;
         const bool in = r < h;
         const float sp = absP * (in ? -3.0f * q * q : 0.f);
-        fsx += (dx * inv) * sp;
-        fsy += (dy * inv) * sp;
-        fsz += (dz * inv) * sp;
+        fsx.add((dx * inv) * sp);
+        fsy.add((dy * inv) * sp);
+        fsz.add((dz * inv) * sp);
         zeroSp += zero ? sp : 0.f;
         anyZero |= zero;
         sumD2s += in ? 6.0f * q : 0.f;
@@ -448,17 +533,17 @@ __global__ __launch_bounds__(kBlock) void k_force_cw(const BlockDesc* __restrict
         });
     if (anyZero) {
         const float3 d = random_direction((float)(ids[i] - S.pBegin));
-        fpx += d.x * zeroSp;
-        fpy += d.y * zeroSp;
-        fpz += d.z * zeroSp;
+        fpx.add(d.x * zeroSp);
+        fpy.add(d.y * zeroSp);
+        fpz.add(d.z * zeroSp);
     }
     // SF:626-638
     const float sP = S.mass / (2.0f * rho) * S.C;
     const float sV = S.mu * S.mass * S.C;
     const float sVs = S.mu * S.mass * S.C * 0.0f;
-    float Fx = fpx * sP + fvx * sV + fsx * sP + (-vi.x * sumD2s) * sVs;
-    float Fy = fpy * sP + fvy * sV + fsy * sP + (-vi.y * sumD2s) * sVs;
-    float Fz = fpz * sP + fvz * sV + fsz * sP + (-vi.z * sumD2s) * sVs;
+    float Fx = fpx.s * sP + fvx.s * sV + fsx.s * sP + (-vi.x * sumD2s) * sVs;
+    float Fy = fpy.s * sP + fvy.s * sV + fsy.s * sP + (-vi.y * sumD2s) * sVs;
+    float Fz = fpz.s * sP + fvz.s * sV + fsz.s * sP + (-vi.z * sumD2s) * sVs;
     if (kTri) {
         const float3 p3 = make_float3(xi.x, xi.y, xi.z);
         for (uint32_t t = 0; t < S.triCount; ++t) {

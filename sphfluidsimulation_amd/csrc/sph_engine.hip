@@ -257,10 +257,14 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     // Density / pressure.
     if ((rc = begin(SPH_STAGE_DENSITY))) return rc;
     const Walk walk = walk_of(h->flags);
+    const bool mixed = (h->flags & SPH_FLAG_FP16_OFFSETS) != 0;  // cell-wave walk only
     if (walk != Walk::Cells) {
 #define SPH_DENS(S, D)                                                                                         \
-    if (walk == Walk::CellWave)                                                                                \
-        hipLaunchKernelGGL((k_density_cw<S, D>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,         \
+    if (walk == Walk::CellWave && mixed)                                                                       \
+        hipLaunchKernelGGL((k_density_cw<S, D, true>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,   \
+                           h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->mask, h->dens);                 \
+    else if (walk == Walk::CellWave)                                                                           \
+        hipLaunchKernelGGL((k_density_cw<S, D, false>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,  \
                            h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->mask, h->dens);                 \
     else if (walk == Walk::Tiles)                                                                              \
         hipLaunchKernelGGL((k_density_tile<S, D>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,       \
@@ -304,8 +308,12 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     if ((rc = begin(SPH_STAGE_FORCE))) return rc;
 #define SPH_FORCE(S, D, T)                                                                                        \
     do {                                                                                                          \
-    if (walk == Walk::CellWave)                                                                                   \
-        hipLaunchKernelGGL((k_force_cw<S, D, T>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,           \
+    if (walk == Walk::CellWave && mixed)                                                                          \
+        hipLaunchKernelGGL((k_force_cw<S, D, T, true>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,     \
+                           h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->mask, h->ids[c], h->tris,  \
+                           dt, h->posT, h->velT, h->tab[c ^ 1], h->acc);                                          \
+    else if (walk == Walk::CellWave)                                                                              \
+        hipLaunchKernelGGL((k_force_cw<S, D, T, false>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,    \
                            h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->mask, h->ids[c], h->tris,  \
                            dt, h->posT, h->velT, h->tab[c ^ 1], h->acc);                                          \
     else if (walk == Walk::Tiles)                                                                                 \

@@ -19,8 +19,8 @@ import pytest
 import oracle as orc
 from helpers import bucket_multisets_equal, check_buckets, expected_starts, max_norm_rel, max_rel
 from sphfluidsimulation_amd import SPHError, SimulationEngineHIP, scenes
-from sphfluidsimulation_amd._lib import (SPH_E_STATE, SPH_FLAG_CELL_WALK, SPH_FLAG_HIP_GRAPH, SPH_FLAG_RECORD_DEBUG,
-                                         SPH_FLAG_RUN_WALK, SPH_FLAG_TILE_WALK)
+from sphfluidsimulation_amd._lib import (SPH_E_STATE, SPH_FLAG_CELL_WALK, SPH_FLAG_FP16_OFFSETS, SPH_FLAG_HIP_GRAPH,
+                                         SPH_FLAG_RECORD_DEBUG, SPH_FLAG_RUN_WALK, SPH_FLAG_TILE_WALK)
 
 pytestmark = pytest.mark.gpu
 
@@ -30,7 +30,8 @@ FP64_DENS_TOL = 1e-4
 
 # The three neighbour-pass variants: LDS-staged bucket tiles (default), Morton
 # runs from HBM, and the reference's cell-by-cell loop from HBM.
-WALKS = [pytest.param(0, id="cellwave"), pytest.param(SPH_FLAG_TILE_WALK, id="tile"),
+WALKS = [pytest.param(0, id="cellwave"), pytest.param(SPH_FLAG_FP16_OFFSETS, id="mixed"),
+         pytest.param(SPH_FLAG_TILE_WALK, id="tile"),
          pytest.param(SPH_FLAG_RUN_WALK, id="runs"),
          pytest.param(SPH_FLAG_CELL_WALK, id="cells")]
 
@@ -138,11 +139,12 @@ def test_one_step_16k(s16k, walk):
     _check_step(s16k, got, ref, 1e-3)
 
 
-def test_density_vs_fp64_16k(s16k):
-    say("fp32 gpu vs fp64 oracle 16k")
+@pytest.mark.parametrize("mode", [pytest.param(0, id="fp32"), pytest.param(SPH_FLAG_FP16_OFFSETS, id="mixed")])
+def test_density_vs_fp64_16k(s16k, mode):
+    say(f"gpu (flags {mode}) vs fp64 oracle 16k")
     sc = s16k
     e = SimulationEngineHIP(0)
-    e.Initialize(sc, flags=SPH_FLAG_RECORD_DEBUG)
+    e.Initialize(sc, flags=SPH_FLAG_RECORD_DEBUG | mode)
     e.Update(1e-3, 1)
     d, a = e.read_debug()
     ref = orc.Oracle(sc.params, sc.pos, sc.vel, sc.static_pos, threads=4, fp64=True, flags=orc.FLAG_RECORD_DEBUG)
@@ -155,6 +157,43 @@ def test_density_vs_fp64_16k(s16k):
     assert arel <= FP64_DENS_TOL
     e.close()
     ref.close()
+
+
+def test_mixed_density_vs_fp64_at_sample_steps():
+    # Mixed precision on a batch of two scenes: at 10 sample steps, the FP64
+    # oracle is seeded with the engine's own state and both take one step;
+    # density, pressure (relative to k rho) and acceleration (infinity-norm
+    # relative) agree to 1e-4.
+    say("mixed precision vs fp64 oracle at 10 seeded sample steps")
+    scs = scenes.batch_scenes(2, 3000, 6000, seed=21)
+    e = SimulationEngineHIP(0)
+    e.Initialize(scs, flags=SPH_FLAG_RECORD_DEBUG | SPH_FLAG_FP16_OFFSETS)
+    worst = {"density": 0.0, "pressure": 0.0, "accel": 0.0}
+    for sample in range(10):
+        pos, vel, _ = e.read()
+        e.Update(1e-3, 1)
+        d, a = e.read_debug()
+        _, _, p = e.read()
+        off = 0
+        for sc in scs:
+            ref = orc.Oracle(sc.params, pos[off:off + sc.n], vel[off:off + sc.n], sc.static_pos, threads=4, fp64=True,
+                             flags=orc.FLAG_RECORD_DEBUG)
+            ref.update(1e-3, 1)
+            rd, ra = ref.read_debug()
+            _, _, rp = ref.read()
+            ref.close()
+            sl = slice(off, off + sc.n)
+            worst["density"] = max(worst["density"], max_rel(d[sl], rd))
+            worst["pressure"] = max(worst["pressure"],
+                                    float(np.max(np.abs(p[sl] - rp) / (sc.params.gasConstant * rd))))
+            worst["accel"] = max(worst["accel"], max_norm_rel(a[sl], ra))
+            off += sc.n
+        e.Update(1e-3, 4)
+    say("worst error vs fp64 over 10 samples: " + ", ".join(f"{k} {v:.2e}" for k, v in worst.items()))
+    assert worst["density"] <= FP64_DENS_TOL
+    assert worst["pressure"] <= FP64_DENS_TOL
+    assert worst["accel"] <= FP64_DENS_TOL
+    e.close()
 
 
 @pytest.mark.parametrize("walk", WALKS)
@@ -278,10 +317,21 @@ def test_debug_smallest_scene(walk):
 
 
 def _occupancy_scene(kind):
+    # aliased: two blocks whose cells share Morton codes, so cell groups hold
+    # particles of cells 1024 apart;
     # crowded: 27 particles per cell on average, so every bucket holds more than
     # the 8 candidates per bucket the cell-group layout keeps in registers;
     # sparse: about 2 per cell, so a wave of 64 particles spans more than the 32
     # cell groups whose bucket tables it keeps at once.
+    if kind == "aliased":
+        # Two copies of a block 1024 cells apart in x: equal Morton codes, so both
+        # copies share buckets and cell groups (hash aliasing, P2).
+        a = scenes.dam_break(12, 12, 12, (1.2, 1.2, 1.2), seed=13, name="aliased-cells")
+        shift = np.float32([1024 * 0.1, 0.0, 0.0])
+        pos = np.concatenate([a.pos, (a.pos + shift).astype(np.float32)])
+        vel = np.concatenate([a.vel, a.vel])
+        walls = np.concatenate([a.static_pos, (a.static_pos + shift).astype(np.float32)])
+        return scenes.Scene(a.params, pos, vel, walls, name="aliased-cells")
     if kind == "crowded":
         return scenes.dam_break(24, 24, 24, (2.4, 2.4, 2.4), spacing=0.1 / 3.0, jitter_frac=0.3, seed=11,
                                 shuffle=True, name="crowded-cells")
@@ -289,7 +339,7 @@ def _occupancy_scene(kind):
                             name="sparse-cells")
 
 
-@pytest.mark.parametrize("kind", ["crowded", "sparse"])
+@pytest.mark.parametrize("kind", ["crowded", "sparse", "aliased"])
 @pytest.mark.parametrize("walk", WALKS)
 def test_cell_occupancy_extremes(kind, walk):
     say(f"{kind} cells (walk flags {walk})")
