@@ -8,5 +8,6 @@ ABI of include/sph_hip.h. This package is the Python host over that ABI
 from ._lib import SPHError, load
 from .engine import SimulationEngineHIP
 from . import scenes
+from . import scene_json
 
-__all__ = ["SPHError", "SimulationEngineHIP", "load", "scenes"]
+__all__ = ["SPHError", "SimulationEngineHIP", "load", "scenes", "scene_json"]

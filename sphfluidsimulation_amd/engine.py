@@ -90,6 +90,10 @@ class SimulationEngineHIP:
                              for s, sc in enumerate(scenes)]
         self._dyn_off = dyn_off.astype(np.int64)
         self._st_off = st_off.astype(np.int64)
+        # Boundary meshes (SimulationEngineCPU.cpp:418: triangles = scene.GetMesh().CreateTriangleArray()).
+        for s, sc in enumerate(scenes):
+            if getattr(sc, "triangles", None) is not None and len(sc.triangles):
+                self.set_triangles(s, sc.triangles)
 
     def Update(self, dt: float, steps: int = 1):
         self._check(self._lib.sph_update(self._h, float(dt), int(steps)))

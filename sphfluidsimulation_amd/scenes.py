@@ -61,6 +61,8 @@ class Scene:
     vel: np.ndarray  # (N, 3) float32
     static_pos: np.ndarray  # (N_s, 3) float32, wall particles in original order
     name: str = ""
+    triangles: np.ndarray | None = None  # (T, 9) float32 boundary mesh (p1, p2, p3), or None
+    other: dict = dataclasses.field(default_factory=dict)  # otherParameters of a scene file
 
     @property
     def n(self) -> int:
@@ -289,6 +291,27 @@ def debug_smallest() -> Scene:
     walls = box_shell((2, 2, 2), 0.2, offset=(-1, -1, -1), jitter=1.0, rng=np.random.default_rng(0))
     params = Params(1.0, 200.0, 0.5, 0.5, 0.0, -9.81, 0.0, 7.0, 1.0)
     return Scene(params, pos, np.zeros_like(pos), walls, name="debugSmallestScene")
+
+
+def sphere_triangles(center, radius: float, n_lat: int = 8, n_lon: int = 12) -> np.ndarray:
+    """A closed UV-sphere mesh (T, 9) float32: n_lon triangles at each pole cap
+    and two per quad between, outward-facing (a boundary obstacle for tests)."""
+    c = np.asarray(center, dtype=np.float64)
+    th = np.linspace(0.0, np.pi, n_lat + 1)
+    ph = np.linspace(0.0, 2.0 * np.pi, n_lon + 1)
+
+    def p(i, j):
+        return c + radius * np.array([np.sin(th[i]) * np.cos(ph[j]), np.cos(th[i]), np.sin(th[i]) * np.sin(ph[j])])
+
+    tris = []
+    for i in range(n_lat):
+        for j in range(n_lon):
+            a, b, cc, d = p(i, j), p(i, j + 1), p(i + 1, j), p(i + 1, j + 1)
+            if i > 0:
+                tris.append(np.concatenate([a, b, cc]))
+            if i < n_lat - 1:
+                tris.append(np.concatenate([b, d, cc]))
+    return np.asarray(tris, dtype=np.float32)
 
 
 def concat(scenes):

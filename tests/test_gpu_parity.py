@@ -383,6 +383,73 @@ def test_autotune_leaves_state_unchanged(s16k):
     ref.close()
 
 
+def _one_step_with_mesh(sc, tris, dt, walk):
+    e = SimulationEngineHIP(0)
+    e.Initialize(sc, flags=SPH_FLAG_RECORD_DEBUG | walk)
+    e.set_triangles(0, tris)
+    ref = orc.Oracle(sc.params, sc.pos, sc.vel, sc.static_pos, threads=4, flags=orc.FLAG_RECORD_DEBUG)
+    ref.set_triangles(tris)
+    e.Update(dt, 1)
+    ref.update(dt, 1)
+    d, a = e.read_debug()
+    rd, ra = ref.read_debug()
+    pos, vel, p = e.read()
+    rpos, rvel, rp = ref.read()
+    e.close()
+    ref.close()
+    return (d, a, pos, vel, p), (rd, ra, rpos, rvel, rp)
+
+
+@pytest.mark.parametrize("walk", WALKS)
+def test_triangle_mesh_one_step(walk):
+    # A sphere mesh (SPHFunctions.cpp:156-223, 636-637) beside the block: every
+    # particle within 2.5 of a triangle feels it.
+    say(f"triangle boundary mesh (walk flags {walk})")
+    sc = scenes.dam_break(16, 16, 16, (1.6, 1.6, 1.6), seed=14, shuffle=True, name="mesh-4k")
+    tris = scenes.sphere_triangles((1.05, 0.4, 0.4), 0.15, n_lat=6, n_lon=10)
+    got, ref = _one_step_with_mesh(sc, tris, 1e-3, walk)
+    _check_step(sc, got, ref, 1e-3)
+
+
+def test_scene_file_with_mesh_one_step(tmp_path):
+    # A scene file in the reference's format with a boundary mesh, through the
+    # loader into the engine (SceneBlueprint.cpp:134-183; SimulationEngineCPU.cpp:418).
+    say("scene file with boundary mesh")
+    tris = scenes.sphere_triangles((0.0, -1.8, 0.0), 0.5, n_lat=8, n_lon=12)
+    with open(tmp_path / "ball.obj", "w") as fh:
+        fh.write("o Ball\n")
+        for t in tris:
+            for k in range(3):
+                fh.write(f"v {t[3 * k]:.6f} {t[3 * k + 1]:.6f} {t[3 * k + 2]:.6f}\n")
+        for f in range(tris.shape[0]):
+            fh.write(f"f {3 * f + 1} {3 * f + 2} {3 * f + 3}\n")
+    doc = {
+        "particleSetBlueprints": [
+            {"layerName": "dynamic", "particleSetBlueprintType": "FilledBox",
+             "properties": {"spawnVolumeSize": [4, 4, 4], "spawnVolumeOffset": [-2, -1, -2], "particlesPerUnit": 20,
+                            "randomOffsetIntensity": 1}},
+        ],
+        "particleBehaviourParameters": {"particleMass": 1.0, "gasConstant": 200, "elasticity": 1, "viscosity": 0.1,
+                                        "gravity": [0, -9.81, 0], "restDensity": 1.0, "maxInteractionDistance": 2},
+        "boundaryMesh": {"path": "ball.obj"},
+    }
+    (tmp_path / "ball.json").write_text(__import__("json").dumps(doc))
+    from sphfluidsimulation_amd import scene_json
+    sc = scene_json.load_scene(str(tmp_path / "ball.json"))
+    assert sc.triangles.shape == tris.shape
+    e = SimulationEngineHIP(0)
+    e.Initialize(sc, flags=SPH_FLAG_RECORD_DEBUG)  # sets the scene's triangles
+    ref = orc.Oracle(sc.params, sc.pos, sc.vel, sc.static_pos, threads=4, flags=orc.FLAG_RECORD_DEBUG)
+    ref.set_triangles(sc.triangles)
+    e.Update(1e-2, 1)
+    ref.update(1e-2, 1)
+    got = e.read_debug() + e.read()
+    want = ref.read_debug() + ref.read()
+    e.close()
+    ref.close()
+    _check_step(sc, got, want, 1e-2)
+
+
 def test_update_before_initialize_and_empty():
     say("error conventions")
     e = SimulationEngineHIP(0)
