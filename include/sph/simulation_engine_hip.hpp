@@ -84,39 +84,67 @@ class SimulationEngine {
 
 class SimulationEngineHIP;
 
-// Readback manager of SimulationEngineHIP (RenderableGPUParticleBufferManager.cpp:256-270
-// reads the current buffer back to the host; this one does so on LockRead).
-class HIPReadbackBufferManager final : public ParticleBufferManager {
+// Buffer manager of SimulationEngineHIP: the HIP form of the reference's
+// renderable GPU buffer manager (RenderableGPUParticleBufferManager.cpp:190-270,
+// OpenCLResourceLock.cpp:73-163). Advance() enqueues, behind the steps already
+// enqueued, a copy of the state into the next of three pinned host buffers and
+// returns at once; LockRead() waits for the newest copy (enqueuing one first if
+// the state changed since) and returns it as 32-byte records; TryLockRead() is
+// the non-blocking form. A locked view stays valid until its slot comes round
+// again, three Advance() calls later.
+class HIPAsyncBufferManager final : public ParticleBufferManager {
   public:
-    explicit HIPReadbackBufferManager(sph_handle h) : h_(h) {}
+    explicit HIPAsyncBufferManager(sph_handle h) : h_(h) {}
     uint64_t GetParticleCount() const override { return sph_particle_count(h_); }
-    const std::vector<DynamicParticle>& LockRead() override {
-        const uint64_t n = GetParticleCount();
-        pos_.resize(3 * n);
-        vel_.resize(3 * n);
-        p_.resize(n);
-        keys_.resize(n);
-        records_.resize(n);
-        if (n) {
-            check(sph_read(h_, pos_.data(), vel_.data(), p_.data()));
-            check(sph_read_keys(h_, keys_.data()));
-        }
-        for (uint64_t i = 0; i < n; ++i)
-            records_[i] = DynamicParticle{{pos_[3 * i], pos_[3 * i + 1], pos_[3 * i + 2]},
-                                          p_[i],
-                                          {vel_[3 * i], vel_[3 * i + 1], vel_[3 * i + 2]},
-                                          keys_[i]};
-        return records_;
+
+    void Advance() {
+        uint32_t slot = 0;
+        check(sph_readback_enqueue(h_, &slot));
+        slot_ = static_cast<int>(slot);
+        stale_ = false;
     }
+    const std::vector<DynamicParticle>& LockRead() override {
+        if (slot_ < 0 || stale_) Advance();
+        return view(static_cast<uint32_t>(slot_));
+    }
+    // Returns false (and leaves `out` alone) while the newest copy is in flight.
+    bool TryLockRead(const std::vector<DynamicParticle>*& out) {
+        if (slot_ < 0 || stale_) Advance();
+        int ready = 0;
+        check(sph_readback_query(h_, static_cast<uint32_t>(slot_), &ready));
+        if (!ready) return false;
+        out = &view(static_cast<uint32_t>(slot_));
+        return true;
+    }
+    double LockedSimulationTime() const { return lockedTime_; }
     void FlushAllOperations() override { check(sph_synchronize(h_)); }
+    void MarkStale() { stale_ = true; }  // the engine enqueued steps
+    void Reset() {
+        slot_ = -1;
+        stale_ = true;
+    }
 
   private:
-    void check(int rc) {
+    const std::vector<DynamicParticle>& view(uint32_t slot) {
+        const float *pos = nullptr, *vel = nullptr, *p = nullptr;
+        const uint32_t* keys = nullptr;
+        uint64_t n = 0;
+        check(sph_readback_lock(h_, slot, &pos, &vel, &p, &keys, &n, &lockedTime_));
+        records_.resize(n);
+        for (uint64_t i = 0; i < n; ++i)
+            records_[i] = DynamicParticle{{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]},
+                                          p[i],
+                                          {vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]},
+                                          keys[i]};
+        return records_;
+    }
+    void check(int rc) const {
         if (rc != SPH_OK) throw std::runtime_error(std::string("sph: ") + sph_last_error(h_));
     }
     sph_handle h_;
-    std::vector<float> pos_, vel_, p_;
-    std::vector<uint32_t> keys_;
+    int slot_ = -1;
+    bool stale_ = true;
+    double lockedTime_ = 0.0;
     std::vector<DynamicParticle> records_;
 };
 
@@ -127,7 +155,7 @@ class SimulationEngineHIP final : public SimulationEngine {
   public:
     explicit SimulationEngineHIP(int device = 0, uint32_t flags = 0) : flags_(flags) {
         if (sph_create(&h_, device) != SPH_OK) throw std::runtime_error("sph_create: no HIP device");
-        manager_ = new HIPReadbackBufferManager(h_);
+        manager_ = new HIPAsyncBufferManager(h_);
     }
     ~SimulationEngineHIP() override {
         delete manager_;
@@ -136,7 +164,10 @@ class SimulationEngineHIP final : public SimulationEngine {
     SimulationEngineHIP(const SimulationEngineHIP&) = delete;
     SimulationEngineHIP& operator=(const SimulationEngineHIP&) = delete;
 
-    void Clear() override { check(sph_clear(h_)); }
+    void Clear() override {
+        check(sph_clear(h_));
+        manager_->Reset();
+    }
 
     void Initialize(const SceneArrays& scene) override {
         const uint64_t n = scene.dynamicPositions.size();
@@ -149,11 +180,16 @@ class SimulationEngineHIP final : public SimulationEngine {
                              flags_));
         if (!scene.triangles.empty())
             check(sph_set_triangles(h_, 0, scene.triangles.data(), scene.triangles.size() / 9));
+        manager_->Reset();
     }
 
-    void Update(float dt, uint32_t steps) override { check(sph_update(h_, dt, steps)); }
+    void Update(float dt, uint32_t steps) override {
+        check(sph_update(h_, dt, steps));
+        manager_->MarkStale();
+    }
     const char* SystemImplementationName() override { return sph_system_implementation_name(); }
     ParticleBufferManager* GetDynamicParticlesBufferManager() override { return manager_; }
+    HIPAsyncBufferManager* GetAsyncBufferManager() { return manager_; }
     float GetSimulationTime() override { return sph_simulation_time(h_); }
 
     sph_handle handle() const { return h_; }
@@ -164,7 +200,7 @@ class SimulationEngineHIP final : public SimulationEngine {
     }
     sph_handle h_ = nullptr;
     uint32_t flags_ = 0;
-    HIPReadbackBufferManager* manager_ = nullptr;
+    HIPAsyncBufferManager* manager_ = nullptr;
 };
 
 }  // namespace SPH

@@ -133,6 +133,25 @@ int sph_set_triangles(sph_handle h, uint32_t scene, const float* tri9, uint64_t 
  * (SimulationEngineCPU.cpp:422-429). */
 int sph_update(sph_handle h, float dt, uint32_t steps);
 
+/* Asynchronous readback, the HIP form of the reference's renderable buffer
+ * manager (RenderableGPUParticleBufferManager.cpp:190-270: a ring of buffers, a
+ * read lock that waits on an event, OpenCLResourceLock.cpp:73-163). The engine
+ * keeps a ring of 3 pinned host buffers.
+ *
+ * sph_readback_enqueue enqueues, after every step already enqueued, a copy of the
+ * state at that point (original particle order) into the next ring slot, and
+ * returns the slot at once; later sph_update calls do not change that copy.
+ * sph_readback_query tells whether the copy has landed (*ready = 1) without
+ * waiting. sph_readback_lock waits for it and returns pointers into the pinned
+ * slot: pos and vel hold 3 floats per particle, pressure and keys one each,
+ * sim_time the simulation time of the copy. The pointers stay valid until the
+ * slot is enqueued again (three enqueues later) or the engine is cleared.
+ * Querying or locking a slot never enqueued returns SPH_E_STATE. */
+int sph_readback_enqueue(sph_handle h, uint32_t* slot);
+int sph_readback_query(sph_handle h, uint32_t slot, int* ready);
+int sph_readback_lock(sph_handle h, uint32_t slot, const float** pos, const float** vel, const float** pressure,
+                      const uint32_t** keys, uint64_t* n, double* sim_time);
+
 /* Waits for all enqueued work (ParticleBufferManager::FlushAllOperations,
  * ParticleBufferManager.h:24). */
 int sph_synchronize(sph_handle h);

@@ -89,6 +89,11 @@ SIGNATURES = {
     "sph_stage_times": (ctypes.c_int, [_P, _F64P, _U64P]),
     "sph_copy_probe": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint32]),
     "sph_autotune": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_uint32, _U32P]),
+    "sph_readback_enqueue": (ctypes.c_int, [_P, _U32P]),
+    "sph_readback_query": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int)]),
+    "sph_readback_lock": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.POINTER(_F32P), ctypes.POINTER(_F32P),
+                                         ctypes.POINTER(_F32P), ctypes.POINTER(_U32P), _U64P,
+                                         ctypes.POINTER(ctypes.c_double)]),
 }
 
 _lib = None

@@ -75,6 +75,18 @@ struct sph_engine {
     bool debugValid = false;
     double simTime = 0.0;
 
+    // Asynchronous readback ring: kReadbackSlots pinned host buffers, each
+    // [pos 3N | vel 3N | pressure N | key N] in original particle order, filled
+    // from one device staging buffer by stream-ordered copies.
+    static constexpr uint32_t kReadbackSlots = 3;
+    float* rbDev = nullptr;  // staging: the host layout, then N slot ids
+    uint64_t rbN = 0;
+    void* rbHost[kReadbackSlots] = {nullptr, nullptr, nullptr};
+    hipEvent_t rbEvent[kReadbackSlots] = {nullptr, nullptr, nullptr};
+    bool rbFilled[kReadbackSlots] = {false, false, false};
+    double rbTime[kReadbackSlots] = {0, 0, 0};
+    uint32_t rbNext = 0;
+
     // hipGraph replay: one graph per starting parity, keyed by dt.
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     float graphDt[2] = {0.f, 0.f};
@@ -150,6 +162,16 @@ void release(sph_engine* h) {
     dfree(h->consOut);
     dfree(h->valBits);
     dfree(h->valIdx);
+    dfree(h->rbDev);
+    for (uint32_t k = 0; k < sph_engine::kReadbackSlots; ++k) {
+        if (h->rbHost[k]) (void)hipHostFree(h->rbHost[k]);
+        h->rbHost[k] = nullptr;
+        if (h->rbEvent[k]) (void)hipEventDestroy(h->rbEvent[k]);
+        h->rbEvent[k] = nullptr;
+        h->rbFilled[k] = false;
+    }
+    h->rbN = 0;
+    h->rbNext = 0;
     h->hScenes.clear();
     h->hBlocks.clear();
     h->hSceneBlocks.clear();
@@ -745,6 +767,85 @@ int sph_read(sph_handle h, float* pos, float* vel, float* pressure) {
         h->err = std::string("sph_read: ") + hipGetErrorString(e);
         return SPH_E_HIP;
     }
+    return SPH_OK;
+}
+
+int sph_readback_enqueue(sph_handle h, uint32_t* slot) {
+    if (!valid(h) || !slot) return SPH_E_INVALID;
+    if (!h->initialized) {
+        h->err = "sph_readback_enqueue: engine not initialized";
+        return SPH_E_STATE;
+    }
+    SPH_HIP(h, hipSetDevice(h->device));
+    const uint64_t N = h->N;
+    const uint64_t words = 8 * N;  // pos 3N, vel 3N, pressure N, key N
+    if (!h->rbEvent[0] || h->rbN != N || (N && !h->rbDev)) {  // first use after initialize
+        dfree(h->rbDev);
+        int rc = dalloc(h, &h->rbDev, words + N);
+        if (rc) return rc;
+        for (uint32_t k = 0; k < sph_engine::kReadbackSlots; ++k) {
+            if (h->rbHost[k]) (void)hipHostFree(h->rbHost[k]);
+            h->rbHost[k] = nullptr;
+            h->rbFilled[k] = false;
+            if (N) SPH_HIP(h, hipHostMalloc(&h->rbHost[k], sizeof(float) * words, hipHostMallocDefault));
+            if (!h->rbEvent[k]) SPH_HIP(h, hipEventCreateWithFlags(&h->rbEvent[k], hipEventDisableTiming));
+        }
+        h->rbN = N;
+        h->rbNext = 0;
+    }
+    const uint32_t k = h->rbNext;
+    if (N) {
+        float* d = h->rbDev;
+        hipLaunchKernelGGL(k_unsort, dim3((uint32_t)((N + 255) / 256)), dim3(256), 0, h->stream, h->pos, h->vel,
+                           h->ids[h->cur], h->origOfSlot, (uint32_t)N, d, d + 3 * N, d + 6 * N,
+                           reinterpret_cast<uint32_t*>(d + 7 * N), reinterpret_cast<uint32_t*>(d + 8 * N));
+        SPH_HIP(h, hipGetLastError());
+        SPH_HIP(h, hipMemcpyAsync(h->rbHost[k], d, sizeof(float) * words, hipMemcpyDeviceToHost, h->stream));
+    }
+    SPH_HIP(h, hipEventRecord(h->rbEvent[k], h->stream));
+    h->rbFilled[k] = true;
+    h->rbTime[k] = h->simTime;
+    h->rbNext = (k + 1) % sph_engine::kReadbackSlots;
+    *slot = k;
+    return SPH_OK;
+}
+
+int sph_readback_query(sph_handle h, uint32_t slot, int* ready) {
+    if (!valid(h) || !ready || slot >= sph_engine::kReadbackSlots) return SPH_E_INVALID;
+    if (!h->rbFilled[slot]) {
+        h->err = "sph_readback_query: slot holds no readback";
+        return SPH_E_STATE;
+    }
+    SPH_HIP(h, hipSetDevice(h->device));
+    const hipError_t e = hipEventQuery(h->rbEvent[slot]);
+    if (e == hipSuccess) {
+        *ready = 1;
+    } else if (e == hipErrorNotReady) {
+        *ready = 0;
+    } else {
+        h->err = std::string("sph_readback_query: ") + hipGetErrorString(e);
+        return SPH_E_HIP;
+    }
+    return SPH_OK;
+}
+
+int sph_readback_lock(sph_handle h, uint32_t slot, const float** pos, const float** vel, const float** pressure,
+                      const uint32_t** keys, uint64_t* n, double* sim_time) {
+    if (!valid(h) || slot >= sph_engine::kReadbackSlots) return SPH_E_INVALID;
+    if (!h->rbFilled[slot]) {
+        h->err = "sph_readback_lock: slot holds no readback";
+        return SPH_E_STATE;
+    }
+    SPH_HIP(h, hipSetDevice(h->device));
+    SPH_HIP(h, hipEventSynchronize(h->rbEvent[slot]));
+    const uint64_t N = h->rbN;
+    const float* b = static_cast<const float*>(h->rbHost[slot]);
+    if (pos) *pos = b;
+    if (vel) *vel = b ? b + 3 * N : nullptr;
+    if (pressure) *pressure = b ? b + 6 * N : nullptr;
+    if (keys) *keys = b ? reinterpret_cast<const uint32_t*>(b + 7 * N) : nullptr;
+    if (n) *n = N;
+    if (sim_time) *sim_time = h->rbTime[slot];
     return SPH_OK;
 }
 

@@ -216,6 +216,38 @@ class SimulationEngineHIP:
         self._check(self._lib.sph_autotune(self._h, ctypes.c_float(dt), int(trial_steps), ctypes.byref(chosen)))
         return self.WALK_NAMES[int(chosen.value)]
 
+    # -- asynchronous readback (RenderableGPUParticleBufferManager.cpp:190-270) --
+    def readback_enqueue(self) -> int:
+        """Enqueue a copy of the state after the steps enqueued so far into the
+        next of 3 pinned slots; returns the slot without waiting."""
+        slot = ctypes.c_uint32(0)
+        self._check(self._lib.sph_readback_enqueue(self._h, ctypes.byref(slot)))
+        return int(slot.value)
+
+    def readback_ready(self, slot: int) -> bool:
+        ready = ctypes.c_int(0)
+        self._check(self._lib.sph_readback_query(self._h, int(slot), ctypes.byref(ready)))
+        return bool(ready.value)
+
+    def readback_lock(self, slot: int):
+        """Wait for a slot; returns copies of (pos, vel, pressure, keys, sim_time)."""
+        fp = [ctypes.POINTER(ctypes.c_float)() for _ in range(3)]
+        kp = ctypes.POINTER(ctypes.c_uint32)()
+        n = ctypes.c_uint64(0)
+        t = ctypes.c_double(0.0)
+        self._check(self._lib.sph_readback_lock(self._h, int(slot), ctypes.byref(fp[0]), ctypes.byref(fp[1]),
+                                                ctypes.byref(fp[2]), ctypes.byref(kp), ctypes.byref(n),
+                                                ctypes.byref(t)))
+        N = int(n.value)
+        if N == 0:
+            z = np.zeros((0, 3), np.float32)
+            return z, z.copy(), np.zeros(0, np.float32), np.zeros(0, np.uint32), float(t.value)
+        pos = np.ctypeslib.as_array(fp[0], shape=(N, 3)).copy()
+        vel = np.ctypeslib.as_array(fp[1], shape=(N, 3)).copy()
+        p = np.ctypeslib.as_array(fp[2], shape=(N,)).copy()
+        keys = np.ctypeslib.as_array(kp, shape=(N,)).copy()
+        return pos, vel, p, keys, float(t.value)
+
     def copy_probe(self, nbytes: int, repeats: int = 1):
         """Plain device copy of nbytes (counter calibration under a profiler)."""
         self._check(self._lib.sph_copy_probe(self._h, int(nbytes), int(repeats)))

@@ -30,10 +30,21 @@ int main() {
         SPH::SimulationEngineHIP engine(0);
         SPH::SimulationEngine& e = engine;
         e.Initialize(scene);
-        for (int i = 0; i < frames; ++i) e.Update(dt, 1);
+        // Frames as the reference's app runs them, with the asynchronous
+        // readback a renderer would use: enqueue a copy each frame, consume it
+        // when it has landed without waiting for the GPU.
+        SPH::HIPAsyncBufferManager* async = engine.GetAsyncBufferManager();
+        unsigned landed = 0;
+        for (int i = 0; i < frames; ++i) {
+            e.Update(dt, 1);
+            async->Advance();
+            const std::vector<SPH::DynamicParticle>* view = nullptr;
+            if (async->TryLockRead(view) && view->size() == n) ++landed;
+        }
         SPH::ParticleBufferManager* m = e.GetDynamicParticlesBufferManager();
         m->FlushAllOperations();
         const auto& recs = m->LockRead();
+        fprintf(stderr, "frames with a landed copy: %u of %d\n", landed, frames);
         printf("%s %llu %.9g\n", e.SystemImplementationName(), (unsigned long long)m->GetParticleCount(),
                (double)e.GetSimulationTime());
         for (const auto& r : recs)

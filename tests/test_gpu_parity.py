@@ -450,6 +450,35 @@ def test_scene_file_with_mesh_one_step(tmp_path):
     _check_step(sc, got, want, 1e-2)
 
 
+def test_async_readback_ring(s16k):
+    say("asynchronous readback ring")
+    e = SimulationEngineHIP(0)
+    e.Initialize(s16k)
+    with pytest.raises(SPHError):
+        e.readback_lock(0)  # nothing enqueued yet
+    e.Update(1e-3, 5)
+    s0 = e.readback_enqueue()
+    want0 = e.read()  # synchronous read of the same state
+    keys0 = e.read_keys()
+    e.Update(1e-3, 5)  # enqueued after the copy: must not change slot s0
+    s1 = e.readback_enqueue()
+    assert s1 == (s0 + 1) % 3
+    pos, vel, p, keys, t = e.readback_lock(s0)
+    np.testing.assert_array_equal(pos, want0[0])
+    np.testing.assert_array_equal(vel, want0[1])
+    np.testing.assert_array_equal(p, want0[2])
+    np.testing.assert_array_equal(keys, keys0)
+    assert t == pytest.approx(5e-3)
+    pos1, _, _, _, t1 = e.readback_lock(s1)
+    np.testing.assert_array_equal(pos1, e.read()[0])
+    assert t1 == pytest.approx(1e-2)
+    assert e.readback_ready(s1)
+    # The ring wraps after three slots.
+    slots = [e.readback_enqueue() for _ in range(3)]
+    assert slots == [(s1 + 1) % 3, (s1 + 2) % 3, s1]
+    e.close()
+
+
 def test_update_before_initialize_and_empty():
     say("error conventions")
     e = SimulationEngineHIP(0)
