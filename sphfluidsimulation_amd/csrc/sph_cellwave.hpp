@@ -86,34 +86,49 @@ __device__ inline uint64_t pick4(const uint64_t (&m)[4], uint32_t r) {
 // the loads feeding them are issued and complete before this point, which keeps
 // the compiler from sinking a batch of loads into the predicated branches that
 // use them (one wait for four loads would become four waits).
+__device__ inline void settle(uint32_t (&j)[2]) { asm volatile("" : "+v"(j[0]), "+v"(j[1])); }
 __device__ inline void settle(uint32_t (&j)[4]) {
     asm volatile("" : "+v"(j[0]), "+v"(j[1]), "+v"(j[2]), "+v"(j[3]));
 }
+__device__ inline void settle(uint32_t (&j)[8]) {
+    asm volatile("" : "+v"(j[0]), "+v"(j[1]), "+v"(j[2]), "+v"(j[3]), "+v"(j[4]), "+v"(j[5]), "+v"(j[6]), "+v"(j[7]));
+}
 
-// Walks the set bits of a neighbour mask four at a time: f(j) gets four
-// candidate indices (bucket start of the bit's bucket + slot), `self` standing in
-// for missing ones, so the four loads of a call are issued together. Rounds are
-// walked in order and bits in ascending order within a round.
-template <class F>
+// Candidates per batch of the mask walk: the density pass loads one float4 per
+// candidate, the force pass two, so the force pass holds fewer in registers.
+#ifndef SPH_CW_BATCH_DENSITY
+#define SPH_CW_BATCH_DENSITY 4
+#endif
+#ifndef SPH_CW_BATCH_FORCE
+#define SPH_CW_BATCH_FORCE 2
+#endif
+constexpr uint32_t kBatchDensity = SPH_CW_BATCH_DENSITY;
+constexpr uint32_t kBatchForce = SPH_CW_BATCH_FORCE;
+
+// Walks the set bits of a neighbour mask B at a time: f(j) gets B candidate
+// indices (bucket start of the bit's bucket + slot), `self` standing in for
+// missing ones, so the B loads of a call are issued together. Rounds are walked
+// in order and bits in ascending order within a round.
+template <uint32_t B, class F>
 __device__ inline void walk_mask(const uint64_t (&m)[4], const uint32_t* start, uint32_t self, F&& f) {
 #pragma unroll 1
     for (uint32_t r = 0; r < 4; ++r) {
         uint64_t mm = pick4(m, r);
         const uint32_t* st = start + 8u * r;
         while (mm) {
-            uint32_t l[4], b[4], j[4];
-            bool ok[4];
+            uint32_t l[B], b[B], j[B];
+            bool ok[B];
 #pragma unroll
-            for (uint32_t u = 0; u < 4; ++u) {
+            for (uint32_t u = 0; u < B; ++u) {
                 ok[u] = mm != 0ull;
                 l[u] = ok[u] ? (uint32_t)__builtin_ctzll(mm) : 0u;
                 mm &= mm - 1ull;
             }
 #pragma unroll
-            for (uint32_t u = 0; u < 4; ++u) b[u] = st[l[u] >> 3];  // valid slots: read unconditionally
+            for (uint32_t u = 0; u < B; ++u) b[u] = st[l[u] >> 3];  // valid slots: read unconditionally
             settle(b);
 #pragma unroll
-            for (uint32_t u = 0; u < 4; ++u) j[u] = ok[u] ? b[u] + (l[u] & 7u) : self;
+            for (uint32_t u = 0; u < B; ++u) j[u] = ok[u] ? b[u] + (l[u] & 7u) : self;
             f(j);
         }
     }
@@ -290,12 +305,12 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
         tuz = h2_bits(pack_h2(uz, uz));
     }
     // Exact contributions of four candidates j (SF:426-453).
-    auto d4 = [&](const uint32_t (&j)[4]) {
-        float4 xj[4];
+    auto d4 = [&](const uint32_t (&j)[kBatchDensity]) {
+        float4 xj[kBatchDensity];
 #pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) xj[u] = ld16(pos, j[u]);
+        for (uint32_t u = 0; u < kBatchDensity; ++u) xj[u] = ld16(pos, j[u]);
 #pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) {
+        for (uint32_t u = 0; u < kBatchDensity; ++u) {
             const float dx = xj[u].x - xi.x, dy = xj[u].y - xi.y, dz = xj[u].z - xi.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
             const float r = __builtin_amdgcn_sqrtf(r2);
@@ -379,15 +394,15 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
             uint64_t m[4];
 #pragma unroll
             for (uint32_t r = 0; r < 4; ++r) m[r] = ((uint64_t)m8.w[2 * r + 1] << 32) | m8.w[2 * r];
-            walk_mask(m, gt.start[wave][slot], i, d4);
-            // Crowded buckets: their particles past the kSlots-th, four at a time.
+            walk_mask<kBatchDensity>(m, gt.start[wave][slot], i, d4);
+            // Crowded buckets: their particles past the kSlots-th, a batch at a time.
             for (; crowded; crowded &= crowded - 1u) {
                 const uint32_t bi = (uint32_t)__builtin_ctz(crowded);
                 const uint32_t s0 = gt.start[wave][slot][bi], n = gt.cnt[wave][slot][bi];
-                for (uint32_t t = kSlots; t < n; t += 4u) {
-                    uint32_t j[4];
+                for (uint32_t t = kSlots; t < n; t += kBatchDensity) {
+                    uint32_t j[kBatchDensity];
 #pragma unroll
-                    for (uint32_t u = 0; u < 4; ++u) j[u] = t + u < n ? s0 + t + u : i;
+                    for (uint32_t u = 0; u < kBatchDensity; ++u) j[u] = t + u < n ? s0 + t + u : i;
                     d4(j);
                 }
             }
@@ -406,8 +421,10 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
     mrow[1] = make_uint4(m8.w[4], m8.w[5], m8.w[6], m8.w[7]);
 }
 
+// Eight waves per SIMD (at most 64 VGPRs) in FP32; the Kahan sums of the mixed
+// mode need more registers, so that mode settles for six.
 template <bool kStatic, bool kDebug, bool kTri, bool kMixed>
-__global__ __launch_bounds__(kBlock) void k_force_cw(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
+__global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
                                                      const SceneDev* __restrict__ scenes,
                                                      const float4* __restrict__ pos, const float4* __restrict__ vel,
                                                      const uint32_t* __restrict__ tab,
@@ -480,15 +497,15 @@ __global__ __launch_bounds__(kBlock) void k_force_cw(const BlockDesc* __restrict
         fvy.add((vj.y - vi.y) * w2);
         fvz.add((vj.z - vi.z) * w2);
     };
-    auto pair4 = [&](const uint32_t (&j)[4]) {
-        float4 xj[4], vj[4];
+    auto pair4 = [&](const uint32_t (&j)[kBatchForce]) {
+        float4 xj[kBatchForce], vj[kBatchForce];
 #pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) {
+        for (uint32_t u = 0; u < kBatchForce; ++u) {
             xj[u] = ld16(pos, j[u]);
             vj[u] = ld16(vel, j[u]);
         }
 #pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) pair_at(j[u], xj[u], vj[u]);
+        for (uint32_t u = 0; u < kBatchForce; ++u) pair_at(j[u], xj[u], vj[u]);
     };
     // One static candidate (SF:590-623): |p_i| for both sides, viscosity scaled by 0.
     auto wall = [&](uint32_t j) {
@@ -513,15 +530,15 @@ __global__ __launch_bounds__(kBlock) void k_force_cw(const BlockDesc* __restrict
         lane, wave, valid, code, cx, cy, cz, S, table, lo, hi, gt,
         [&](uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, auto&& between) { between(); },
         [&](uint32_t slot, uint32_t crowded) {
-            walk_mask(m, gt.start[wave][slot], i, pair4);
-            // Crowded buckets: their particles past the kSlots-th, four at a time.
+            walk_mask<kBatchForce>(m, gt.start[wave][slot], i, pair4);
+            // Crowded buckets: their particles past the kSlots-th, a batch at a time.
             for (; crowded; crowded &= crowded - 1u) {
                 const uint32_t bi = (uint32_t)__builtin_ctz(crowded);
                 const uint32_t s0 = gt.start[wave][slot][bi], n = gt.cnt[wave][slot][bi];
-                for (uint32_t t = kSlots; t < n; t += 4u) {
-                    uint32_t j[4];
+                for (uint32_t t = kSlots; t < n; t += kBatchForce) {
+                    uint32_t j[kBatchForce];
 #pragma unroll
-                    for (uint32_t u = 0; u < 4; ++u) j[u] = t + u < n ? s0 + t + u : i;
+                    for (uint32_t u = 0; u < kBatchForce; ++u) j[u] = t + u < n ? s0 + t + u : i;
                     pair4(j);
                 }
             }
