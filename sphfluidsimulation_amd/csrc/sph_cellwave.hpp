@@ -18,8 +18,9 @@
 // with more than 8 particles (crowded cells) leave their extra particles out of
 // the layout; each lane of such a group tests them directly, so every bucket is
 // visited exactly as often as in the reference, colliding buckets included.
-// Static (wall) neighbours are walked per particle through merged Morton runs
-// of the static table afterwards.
+// Static (wall) neighbours: each group reads the wall-cell bitmap bits of its 27
+// neighbour cells with its tables, and only lanes of groups near walls walk the
+// static buckets (wall_visits).
 // SF = SPHFluidSimulation/source/SPH/Kernels/SPHFunctions.cpp.
 #pragma once
 
@@ -194,28 +195,66 @@ __device__ inline RoundCand round_candidate(uint32_t r, uint32_t lane, uint32_t 
 // raw table entries of neighbour cell `lane` in the reference's x-y-z visit
 // order (SF:397-404).
 struct GroupFetch {
-    uint32_t g0, g1, tb, te;
+    uint32_t g0, g1, tb, te;  // lanes g0..g1-1; dynamic table entries of neighbour cell `lane`
+    uint32_t wword;           // the word of the wall-cell bitmap holding that cell's bit
 };
 
+// Neighbour cell `lane` (lanes past 26 repeat cell 26) of the group whose first
+// lane is g0, in the reference's x-y-z visit order (SF:397-404).
+__device__ inline void neighbour_cell(uint32_t lane, uint32_t g0, uint32_t cx, uint32_t cy, uint32_t cz,
+                                      uint32_t& nx, uint32_t& ny, uint32_t& nz) {
+    const uint32_t gx = (uint32_t)__builtin_amdgcn_readlane((int)cx, (int)g0);
+    const uint32_t gy = (uint32_t)__builtin_amdgcn_readlane((int)cy, (int)g0);
+    const uint32_t gz = (uint32_t)__builtin_amdgcn_readlane((int)cz, (int)g0);
+    const uint32_t n = lane < 27u ? lane : 26u;
+    nx = gx + n / 9u - 1u;
+    ny = gy + (n / 3u) % 3u - 1u;
+    nz = gz + n % 3u - 1u;
+}
+
+template <bool kStatic>
 __device__ inline GroupFetch fetch_group(uint64_t& heads, uint64_t stops, uint32_t lane, uint32_t cx, uint32_t cy,
                                          uint32_t cz, const SceneDev& S, const uint32_t* __restrict__ table,
-                                         uint32_t lo) {
+                                         const uint32_t* __restrict__ wallMap) {
     GroupFetch f;
     f.g0 = (uint32_t)__builtin_ctzll(heads);
     heads &= heads - 1ull;
     const uint64_t above = f.g0 < 63u ? (stops & ~((2ull << f.g0) - 1ull)) : 0ull;
     f.g1 = above ? (uint32_t)__builtin_ctzll(above) : 64u;
-    const uint32_t gx = (uint32_t)__builtin_amdgcn_readlane((int)cx, (int)f.g0);
-    const uint32_t gy = (uint32_t)__builtin_amdgcn_readlane((int)cy, (int)f.g0);
-    const uint32_t gz = (uint32_t)__builtin_amdgcn_readlane((int)cz, (int)f.g0);
+    uint32_t nx, ny, nz;
+    neighbour_cell(lane, f.g0, cx, cy, cz, nx, ny, nz);
     // Every lane reads (lanes past 26 repeat cell 26; the reader ignores them), so
     // the reads are not in a branch and a wait for later loads need not cover them.
-    const uint32_t n = lane < 27u ? lane : 26u;
-    const uint32_t a = n / 9u, b = (n / 3u) % 3u, c = n % 3u;
-    const uint32_t bk = fastmod(morton3(gx + a - 1u, gy + b - 1u, gz + c - 1u), S.magicM, S.M);
+    const uint32_t bk = fastmod(morton3(nx, ny, nz), S.magicM, S.M);
     f.tb = table[bk];
     f.te = table[bk + 1];
+    f.wword = kStatic ? wallMap[S.wmBegin + ((cell_hash(nx, ny, nz) >> 5) & S.wmMask)] : 0u;
     return f;
+}
+
+// Neighbour cells (bits 0..26, visit order) whose static bucket must be walked
+// for the group whose first lane is g0: every cell whose wall-bitmap bit is set
+// (the bitmap has the bit of every cell holding walls, so a clear bit means no
+// walls there), and every cell sharing a static bucket with one of those, since
+// the reference walks a bucket once per neighbour cell mapping to it
+// (SF:455-496). A skipped bucket holds only walls of cells outside the 27,
+// farther than h, which the reference tests and drops: skipping them changes no
+// sum.
+__device__ inline uint32_t wall_visits(uint32_t wword, uint32_t lane, uint32_t g0, uint32_t cx, uint32_t cy,
+                                       uint32_t cz, const SceneDev& S) {
+    if (S.sCount == 0u) return 0u;
+    uint32_t nx, ny, nz;
+    neighbour_cell(lane, g0, cx, cy, cz, nx, ny, nz);
+    const bool hit = lane < 27u && ((wword >> (cell_hash(nx, ny, nz) & 31u)) & 1u);
+    uint32_t hits = (uint32_t)ballot64(hit);
+    if (hits == 0u) return 0u;
+    const uint32_t sb = fastmod(morton3(nx, ny, nz), S.magicMs, S.Ms);
+    uint32_t visits = 0u;
+    for (; hits; hits &= hits - 1u) {
+        const uint32_t sbn = (uint32_t)__builtin_amdgcn_readlane((int)sb, (int)__builtin_ctz(hits));
+        visits |= (uint32_t)ballot64(lane < 27u && sb == sbn);
+    }
+    return visits;
 }
 
 // Iterates the groups of a wave in batches:
@@ -224,19 +263,21 @@ __device__ inline GroupFetch fetch_group(uint64_t& heads, uint64_t stops, uint32
 // count of its neighbour buckets) and calls between() once, after issuing its
 // own loads: between() issues the next group's table reads (unconditionally, so
 // the wait for the group's own loads never has to include them), so they
-// overlap the group's work. Then phase2(slot, crowded) runs on the lanes of the batch's
-// groups with `slot` their group's table slot and `crowded` the bit set of its
-// buckets holding more than kSlots particles.
-template <class P1, class P2>
+// overlap the group's work. Then phase2(slot, crowded) runs on the lanes of the
+// batch's groups with `slot` their group's table slot and `crowded` the bit set
+// of its buckets holding more than kSlots particles. myWalls receives, on every
+// lane of a group, the static buckets its particles walk (wall_visits).
+template <bool kStatic, class P1, class P2>
 __device__ inline void for_each_group(uint32_t lane, uint32_t wave, bool valid, uint32_t code, uint32_t cx,
                                       uint32_t cy, uint32_t cz, const SceneDev& S, const uint32_t* __restrict__ table,
-                                      uint32_t lo, uint32_t hi, GroupTables& gt, P1&& phase1, P2&& phase2) {
+                                      const uint32_t* __restrict__ wallMap, uint32_t lo, uint32_t hi, GroupTables& gt,
+                                      uint32_t& myWalls, P1&& phase1, P2&& phase2) {
     const uint32_t prevCode = (uint32_t)__shfl_up((int)code, 1);
     uint64_t heads = ballot64(valid && (lane == 0u || prevCode != code));
     const uint64_t stops = heads | ~ballot64(valid);  // a group ends at the next head or invalid lane
     while (heads) {
         uint32_t mySlot = 0xFFFFFFFFu, myCrowded = 0u;
-        GroupFetch cur = fetch_group(heads, stops, lane, cx, cy, cz, S, table, lo);
+        GroupFetch cur = fetch_group<kStatic>(heads, stops, lane, cx, cy, cz, S, table, wallMap);
         for (uint32_t slot = 0;; ++slot) {
             const uint32_t s0 = cur.tb > lo ? cur.tb : lo, e0 = cur.te < hi ? cur.te : hi;
             const uint32_t start27 = s0, cnt27 = lane < 27u && e0 > s0 ? e0 - s0 : 0u;  // empty past lane 26
@@ -245,15 +286,17 @@ __device__ inline void for_each_group(uint32_t lane, uint32_t wave, bool valid, 
                 gt.cnt[wave][slot][lane] = cnt27;
             }
             const uint32_t crowded = (uint32_t)ballot64(cnt27 > kSlots);  // buckets 0..26
+            const uint32_t walls = kStatic ? wall_visits(cur.wword, lane, cur.g0, cx, cy, cz, S) : 0u;
             if (lane >= cur.g0 && lane < cur.g1) {
                 mySlot = slot;
                 myCrowded = crowded;
+                myWalls = walls;
             }
             const bool more = heads != 0ull && slot + 1u < kGroupBatch;
             GroupFetch nxt;
             uint64_t rest = more ? heads : 1ull;  // at the batch end: a harmless read of group 0's table
             phase1(slot, cur.g0, cur.g1, start27, cnt27,
-                   [&]() { nxt = fetch_group(rest, stops, lane, cx, cy, cz, S, table, lo); });
+                   [&]() { nxt = fetch_group<kStatic>(rest, stops, lane, cx, cy, cz, S, table, wallMap); });
             if (!more) break;
             heads = rest;
             cur = nxt;
@@ -264,12 +307,31 @@ __device__ inline void for_each_group(uint32_t lane, uint32_t wave, bool valid, 
     }
 }
 
+// Calls f(j) for the static particles of the neighbour cells whose bits are set
+// in `walls`, cell by cell in the reference's visit order (SF:455-496 density,
+// SF:590-623 forces), each bucket's particles in table order, a bucket shared by
+// two neighbour cells once per cell.
+template <class F>
+__device__ inline void walk_walls(uint32_t walls, uint32_t cx, uint32_t cy, uint32_t cz, const SceneDev& S,
+                                  const uint32_t* __restrict__ stable, F&& f) {
+    const uint32_t slo = S.sBegin, shi = S.sBegin + S.sCount;
+    for (; walls; walls &= walls - 1u) {
+        const uint32_t n = (uint32_t)__builtin_ctz(walls);
+        const uint32_t sk = fastmod(morton3(cx + n / 9u - 1u, cy + (n / 3u) % 3u - 1u, cz + n % 3u - 1u), S.magicMs,
+                                    S.Ms);
+        const uint32_t sb = stable[sk], se = stable[sk + 1];
+        const uint32_t s0 = sb > slo ? sb : slo, e0 = se < shi ? se : shi;
+        for (uint32_t j = s0; j < e0; ++j) f(j);
+    }
+}
+
 template <bool kStatic, bool kDebug, bool kMixed>
 __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
                                                        const SceneDev* __restrict__ scenes, float4* pos,
                                                        const uint32_t* __restrict__ tab,
                                                        const float4* __restrict__ spos,
                                                        const uint32_t* __restrict__ stab,
+                                                       const uint32_t* __restrict__ wallMap,
                                                        uint32_t* __restrict__ mask, float* __restrict__ dens) {
     __shared__ GroupTables gt;
     const uint32_t tile = xcd_tile(blockIdx.x, numBlocks);
@@ -329,8 +391,9 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
             sumSt.add((r < h) ? q * q * q : 0.f);
         }
     };
-    for_each_group(
-        lane, wave, valid, code, cx, cy, cz, S, table, lo, hi, gt,
+    uint32_t walls = 0u;  // static buckets this lane walks
+    for_each_group<kStatic>(
+        lane, wave, valid, code, cx, cy, cz, S, table, wallMap, lo, hi, gt, walls,
         [&](uint32_t, uint32_t g0, uint32_t g1, uint32_t start27, uint32_t cnt27, auto&& between) {
             // The particle itself passes its own test (r = 0); d4 skips it. Empty
             // slots are masked out with the (wave-uniform) slot masks.
@@ -408,10 +471,7 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
             }
         });
     if (!valid) return;
-    if (kStatic && S.sCount)
-        walk_runs(cx, cy, cz, S.Ms, S.magicMs, stable, S.sBegin, S.sBegin + S.sCount, [&](uint32_t s0, uint32_t e0) {
-            for (uint32_t j = s0; j < e0; ++j) dWall(j);
-        });
+    if (kStatic) walk_walls(walls, cx, cy, cz, S, stable, dWall);
     const float rho = S.selfDensity + (sumDyn.s * S.mass + sumSt.s * S.mass) * S.C;
     // Only this lane writes pos[i].w; no lane of this pass reads any .w.
     reinterpret_cast<float*>(pos)[4ull * i + 3] = S.k * (rho - S.rho0);
@@ -430,6 +490,7 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const Block
                                                      const uint32_t* __restrict__ tab,
                                                      const float4* __restrict__ spos,
                                                      const uint32_t* __restrict__ stab,
+                                                     const uint32_t* __restrict__ wallMap,
                                                      const uint32_t* __restrict__ mask,
                                                      const uint32_t* __restrict__ ids, const float* __restrict__ tris,
                                                      float dt, float4* __restrict__ posT, float4* __restrict__ velT,
@@ -526,8 +587,9 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const Block
         anyZero |= zero;
         sumD2s += in ? 6.0f * q : 0.f;
     };
-    for_each_group(
-        lane, wave, valid, code, cx, cy, cz, S, table, lo, hi, gt,
+    uint32_t walls = 0u;  // static buckets this lane walks
+    for_each_group<kStatic>(
+        lane, wave, valid, code, cx, cy, cz, S, table, wallMap, lo, hi, gt, walls,
         [&](uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, auto&& between) { between(); },
         [&](uint32_t slot, uint32_t crowded) {
             walk_mask<kBatchForce>(m, gt.start[wave][slot], i, pair4);
@@ -544,10 +606,7 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const Block
             }
         });
     if (!valid) return;
-    if (kStatic && S.sCount)
-        walk_runs(cx, cy, cz, S.Ms, S.magicMs, stable, S.sBegin, S.sBegin + S.sCount, [&](uint32_t s0, uint32_t e0) {
-            for (uint32_t j = s0; j < e0; ++j) wall(j);
-        });
+    if (kStatic) walk_walls(walls, cx, cy, cz, S, stable, wall);
     if (anyZero) {
         const float3 d = random_direction((float)(ids[i] - S.pBegin));
         fpx.add(d.x * zeroSp);

@@ -62,6 +62,8 @@ struct sph_engine {
     uint32_t* ticket = nullptr;
     float4* spos = nullptr;
     uint32_t* stab = nullptr;
+    uint32_t* wallMap = nullptr;  // per scene: bit cell_hash(c) set for every cell c holding walls
+    uint64_t wallWords = 0;
     uint32_t* staticScene = nullptr;
     float* tris = nullptr;
     float* dens = nullptr;
@@ -153,6 +155,8 @@ void release(sph_engine* h) {
     dfree(h->ticket);
     dfree(h->spos);
     dfree(h->stab);
+    dfree(h->wallMap);
+    h->wallWords = 0;
     dfree(h->staticScene);
     dfree(h->tris);
     dfree(h->dens);
@@ -284,10 +288,10 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
 #define SPH_DENS(S, D)                                                                                         \
     if (walk == Walk::CellWave && mixed)                                                                       \
         hipLaunchKernelGGL((k_density_cw<S, D, true>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,   \
-                           h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->mask, h->dens);                 \
+                           h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->wallMap, h->mask, h->dens);     \
     else if (walk == Walk::CellWave)                                                                           \
         hipLaunchKernelGGL((k_density_cw<S, D, false>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,  \
-                           h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->mask, h->dens);                 \
+                           h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->wallMap, h->mask, h->dens);     \
     else if (walk == Walk::Tiles)                                                                              \
         hipLaunchKernelGGL((k_density_tile<S, D>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,       \
                            h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->mask, h->dens);                 \
@@ -332,11 +336,13 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     do {                                                                                                          \
     if (walk == Walk::CellWave && mixed)                                                                          \
         hipLaunchKernelGGL((k_force_cw<S, D, T, true>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,     \
-                           h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->mask, h->ids[c], h->tris,  \
+                           h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->wallMap, h->mask,          \
+                           h->ids[c], h->tris,                                                                    \
                            dt, h->posT, h->velT, h->tab[c ^ 1], h->acc);                                          \
     else if (walk == Walk::CellWave)                                                                              \
         hipLaunchKernelGGL((k_force_cw<S, D, T, false>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,    \
-                           h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->mask, h->ids[c], h->tris,  \
+                           h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->wallMap, h->mask,          \
+                           h->ids[c], h->tris,                                                                    \
                            dt, h->posT, h->velT, h->tab[c ^ 1], h->acc);                                          \
     else if (walk == Walk::Tiles)                                                                                 \
         hipLaunchKernelGGL((k_force_tile<S, D, T>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,         \
@@ -501,7 +507,7 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
     h->L = L;
 
     std::vector<float4> hPos(N), hVel(N), hSpos(Ns);
-    std::vector<uint32_t> hIds(N), hOrig(N), hTab(L), hStab, hStaticScene(Ns);
+    std::vector<uint32_t> hIds(N), hOrig(N), hTab(L), hStab, hStaticScene(Ns), hWallMap;
     h->hStaticIds.assign(Ns, 0);
     h->hScenes.resize(n_scenes);
     h->hSceneBlocks.assign(n_scenes + 1, 0);
@@ -548,6 +554,19 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
             }
             for (uint64_t b = 0; b <= sn; ++b) hStab.push_back((uint32_t)(s0 + table[b]));
             stBegin += sn + 1;
+            // Wall-cell bitmap, 32 bits per wall particle rounded up to a power of
+            // two in words: the cell-wave passes skip the static buckets of
+            // neighbour cells that cannot hold a wall within h.
+            uint64_t words = 32;
+            while (words < sn) words *= 2;
+            S.wmBegin = (uint32_t)hWallMap.size();
+            S.wmMask = (uint32_t)(words - 1);
+            hWallMap.resize(hWallMap.size() + words, 0u);
+            for (uint64_t i = 0; i < sn; ++i) {
+                const float* q = static_pos + 3 * (s0 + i);
+                const uint32_t hc = cell_hash(cell_coord(q[0], S.h), cell_coord(q[1], S.h), cell_coord(q[2], S.h));
+                hWallMap[S.wmBegin + ((hc >> 5) & S.wmMask)] |= 1u << (hc & 31u);
+            }
         }
         // Dynamic: M = 2N (SimulationEngineCPU.cpp:486-487), one-thread order.
         if (pn) {
@@ -608,6 +627,8 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
     SPH_ALLOC(h->ticket, 1);
     SPH_ALLOC(h->spos, std::max<uint64_t>(Ns, 1));
     SPH_ALLOC(h->stab, std::max<uint64_t>(h->Ls, 1));
+    h->wallWords = hWallMap.size();
+    SPH_ALLOC(h->wallMap, std::max<uint64_t>(h->wallWords, 1));
     SPH_ALLOC(h->staticScene, std::max<uint64_t>(Ns, 1));
     SPH_ALLOC(h->consPartial, 8ull * std::max<uint32_t>(h->numBlocks, 1));
     SPH_ALLOC(h->consOut, 8ull * n_scenes);
@@ -641,6 +662,7 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
     if (Ns) {
         SPH_HIP(h, hipMemcpy(h->spos, hSpos.data(), sizeof(float4) * Ns, hipMemcpyHostToDevice));
         SPH_HIP(h, hipMemcpy(h->stab, hStab.data(), sizeof(uint32_t) * h->Ls, hipMemcpyHostToDevice));
+        SPH_HIP(h, hipMemcpy(h->wallMap, hWallMap.data(), sizeof(uint32_t) * h->wallWords, hipMemcpyHostToDevice));
         SPH_HIP(h, hipMemcpy(h->staticScene, hStaticScene.data(), sizeof(uint32_t) * Ns, hipMemcpyHostToDevice));
     }
     // Validation index lists: dynamic first/last entries per scene, then static.

@@ -35,7 +35,8 @@ struct alignas(16) SceneDev {
     uint32_t sBegin, sCount;              // static particles [sBegin, sBegin + sCount)
     uint32_t stBegin, Ms;                 // static table [stBegin, stBegin + Ms]
     uint64_t magicMs;
-    uint32_t triBegin, triCount, pad1, pad2;
+    uint32_t triBegin, triCount;
+    uint32_t wmBegin, wmMask;  // wall-cell bitmap: words [wmBegin, wmBegin + wmMask]
 };
 
 struct alignas(16) BlockDesc {

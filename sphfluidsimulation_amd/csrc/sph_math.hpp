@@ -16,6 +16,19 @@ __host__ __device__ inline uint32_t cell_coord(float x, float h) {
     return (uint32_t)(int32_t)floorf(x / h);
 }
 
+// Hash of a full cell coordinate (the 32-bit wrap of the signed cell index per
+// axis), for the wall-cell bitmap: a multiplicative mix of the three
+// coordinates followed by xor-shift-multiply rounds.
+__host__ __device__ inline uint32_t cell_hash(uint32_t cx, uint32_t cy, uint32_t cz) {
+    uint32_t h = cx * 0x9E3779B1u ^ cy * 0x85EBCA77u ^ cz * 0xC2B2AE3Du;
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    h *= 0x297A2D39u;
+    h ^= h >> 15;
+    return h;
+}
+
 // SF:79-86 (ExpandBits): the low 10 bits of v, bit k moved to bit 3k.
 __host__ __device__ inline uint32_t spread3(uint32_t v) {
     v &= 0x3FFu;
