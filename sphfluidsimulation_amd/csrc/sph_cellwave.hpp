@@ -307,21 +307,37 @@ __device__ inline void for_each_group(uint32_t lane, uint32_t wave, bool valid, 
     }
 }
 
-// Calls f(j) for the static particles of the neighbour cells whose bits are set
-// in `walls`, cell by cell in the reference's visit order (SF:455-496 density,
+// Walks the static particles of the neighbour cells whose bits are set in
+// `walls`, cell by cell in the reference's visit order (SF:455-496 density,
 // SF:590-623 forces), each bucket's particles in table order, a bucket shared by
-// two neighbour cells once per cell.
-template <class F>
+// two neighbour cells once per cell. f(j0, e) takes the batch j0 .. min(j0 + B,
+// e) - 1 of one bucket, so its B loads are in flight together; the next
+// bucket's table entries are read before the current bucket's particles.
+template <uint32_t B, class F>
 __device__ inline void walk_walls(uint32_t walls, uint32_t cx, uint32_t cy, uint32_t cz, const SceneDev& S,
                                   const uint32_t* __restrict__ stable, F&& f) {
+    if (!walls) return;
     const uint32_t slo = S.sBegin, shi = S.sBegin + S.sCount;
-    for (; walls; walls &= walls - 1u) {
-        const uint32_t n = (uint32_t)__builtin_ctz(walls);
+    auto range = [&](uint32_t n, uint32_t& s0, uint32_t& e0) {
         const uint32_t sk = fastmod(morton3(cx + n / 9u - 1u, cy + (n / 3u) % 3u - 1u, cz + n % 3u - 1u), S.magicMs,
                                     S.Ms);
-        const uint32_t sb = stable[sk], se = stable[sk + 1];
-        const uint32_t s0 = sb > slo ? sb : slo, e0 = se < shi ? se : shi;
-        for (uint32_t j = s0; j < e0; ++j) f(j);
+        s0 = stable[sk];
+        e0 = stable[sk + 1];
+    };
+    uint32_t n = (uint32_t)__builtin_ctz(walls);
+    walls &= walls - 1u;
+    uint32_t s0, e0;
+    range(n, s0, e0);
+    for (;;) {
+        const bool more = walls != 0u;
+        uint32_t s1, e1;
+        range(more ? (uint32_t)__builtin_ctz(walls) : n, s1, e1);  // the last round re-reads a valid entry
+        walls &= walls - 1u;
+        const uint32_t a = s0 > slo ? s0 : slo, b = e0 < shi ? e0 : shi;
+        for (uint32_t j = a; j < b; j += B) f(j, b);
+        if (!more) break;
+        s0 = s1;
+        e0 = e1;
     }
 }
 
@@ -381,14 +397,17 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
         }
     };
     // Static neighbours (SF:455-496).
-    auto dWall = [&](uint32_t j) {
-        const float4 xj = ld16(spos, j);
-        const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 <= h2) {
+    auto dWalls = [&](uint32_t j0, uint32_t e) {
+        float4 xj[kBatchDensity];
+#pragma unroll
+        for (uint32_t u = 0; u < kBatchDensity; ++u) xj[u] = ld16(spos, j0 + u < e ? j0 + u : j0);
+#pragma unroll
+        for (uint32_t u = 0; u < kBatchDensity; ++u) {
+            const float dx = xj[u].x - xi.x, dy = xj[u].y - xi.y, dz = xj[u].z - xi.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
             const float r = __builtin_amdgcn_sqrtf(r2);
             const float q = h - r;
-            sumSt.add((r < h) ? q * q * q : 0.f);
+            if (j0 + u < e && r2 <= h2) sumSt.add((r < h) ? q * q * q : 0.f);
         }
     };
     uint32_t walls = 0u;  // static buckets this lane walks
@@ -471,7 +490,7 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
             }
         });
     if (!valid) return;
-    if (kStatic) walk_walls(walls, cx, cy, cz, S, stable, dWall);
+    if (kStatic) walk_walls<kBatchDensity>(walls, cx, cy, cz, S, stable, dWalls);
     const float rho = S.selfDensity + (sumDyn.s * S.mass + sumSt.s * S.mass) * S.C;
     // Only this lane writes pos[i].w; no lane of this pass reads any .w.
     reinterpret_cast<float*>(pos)[4ull * i + 3] = S.k * (rho - S.rho0);
@@ -569,11 +588,10 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const Block
         for (uint32_t u = 0; u < kBatchForce; ++u) pair_at(j[u], xj[u], vj[u]);
     };
     // One static candidate (SF:590-623): |p_i| for both sides, viscosity scaled by 0.
-    auto wall = [&](uint32_t j) {
-        const float4 xj = ld16(spos, j);
+    auto wall_at = [&](const float4 xj, bool live) {
         const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
         const float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 > h2) return;
+        if (!live || r2 > h2) return;
         const float r = __builtin_amdgcn_sqrtf(r2);
         const bool zero = r == 0.f;
         const float inv = zero ? 0.f : __builtin_amdgcn_rcpf(r);
@@ -586,6 +604,13 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const Block
         zeroSp += zero ? sp : 0.f;
         anyZero |= zero;
         sumD2s += in ? 6.0f * q : 0.f;
+    };
+    auto walls4 = [&](uint32_t j0, uint32_t e) {
+        float4 xj[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) xj[u] = ld16(spos, j0 + u < e ? j0 + u : j0);
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) wall_at(xj[u], j0 + u < e);
     };
     uint32_t walls = 0u;  // static buckets this lane walks
     for_each_group<kStatic>(
@@ -606,7 +631,7 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const Block
             }
         });
     if (!valid) return;
-    if (kStatic) walk_walls(walls, cx, cy, cz, S, stable, wall);
+    if (kStatic) walk_walls<4>(walls, cx, cy, cz, S, stable, walls4);
     if (anyZero) {
         const float3 d = random_direction((float)(ids[i] - S.pBegin));
         fpx.add(d.x * zeroSp);
