@@ -341,6 +341,57 @@ __device__ inline void walk_walls(uint32_t walls, uint32_t cx, uint32_t cy, uint
     }
 }
 
+// Launch order of the cell-wave passes' workgroups. A workgroup with particles
+// near walls (marked in `heavy` by the density pass) takes several times as long
+// as one in the interior, and with few workgroups per SIMD the last of them set
+// the pass's duration. So the next step launches them first: tile t of XCD
+// chunk k (xcd_tile) runs block order[t], and within each chunk the marked
+// blocks come first, in block order, then the rest. One workgroup per chunk;
+// clears the marks. Any permutation within the chunks gives the same results;
+// this one only changes when each block runs.
+__global__ __launch_bounds__(256) void k_tile_order(uint32_t* __restrict__ heavy, uint32_t numBlocks,
+                                                    uint32_t* __restrict__ order) {
+    __shared__ uint32_t sWave[4];
+    __shared__ uint32_t sTotal;
+    const uint32_t chunk = (numBlocks + 7u) >> 3;
+    const uint32_t base = blockIdx.x * chunk;
+    const uint32_t valid = base < numBlocks ? min(chunk, numBlocks - base) : 0u;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    // Pass 1: how many marked blocks the chunk holds.
+    uint32_t count = 0;
+    for (uint32_t off = 0; off < valid; off += 256u) {
+        const bool marked = off + tid < valid && heavy[base + off + tid] != 0u;
+        count += (uint32_t)__popcll(__ballot(marked));
+    }
+    if (lane == 0) sWave[wave] = count;
+    __syncthreads();
+    if (tid == 0) sTotal = sWave[0] + sWave[1] + sWave[2] + sWave[3];
+    __syncthreads();
+    const uint32_t marks = sTotal;
+    // Pass 2: stable partition, marked blocks first.
+    uint32_t markedBefore = 0;
+    for (uint32_t off = 0; off < valid; off += 256u) {
+        const uint32_t b = base + off + tid;
+        const bool in = off + tid < valid;
+        const bool marked = in && heavy[b] != 0u;
+        const uint64_t ball = __ballot(marked);
+        const uint32_t below = (uint32_t)__popcll(ball & ((1ull << lane) - 1ull));
+        __syncthreads();
+        if (lane == 0) sWave[wave] = (uint32_t)__popcll(ball);
+        __syncthreads();
+        uint32_t waveOff = 0;
+        for (uint32_t w = 0; w < wave; ++w) waveOff += sWave[w];
+        const uint32_t tileMarks = sWave[0] + sWave[1] + sWave[2] + sWave[3];
+        if (in) {
+            const uint32_t mk = markedBefore + waveOff + below;  // marked blocks before this one
+            const uint32_t pos = marked ? mk : marks + (off + tid - mk);
+            order[base + pos] = b;
+            heavy[b] = 0u;
+        }
+        markedBefore += tileMarks;
+    }
+}
+
 template <bool kStatic, bool kDebug, bool kMixed>
 __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
                                                        const SceneDev* __restrict__ scenes, float4* pos,
@@ -348,11 +399,14 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
                                                        const float4* __restrict__ spos,
                                                        const uint32_t* __restrict__ stab,
                                                        const uint32_t* __restrict__ wallMap,
+                                                       const uint32_t* __restrict__ order,
+                                                       uint32_t* __restrict__ heavy,
                                                        uint32_t* __restrict__ mask, float* __restrict__ dens) {
     __shared__ GroupTables gt;
     const uint32_t tile = xcd_tile(blockIdx.x, numBlocks);
     if (tile >= numBlocks) return;
-    const BlockDesc bd = blocks[tile];
+    const uint32_t blk = order[tile];
+    const BlockDesc bd = blocks[blk];
     const SceneDev& S = scenes[bd.scene];
     const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
     const uint32_t wbegin = bd.begin + 64u * wave;
@@ -489,6 +543,7 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
                 }
             }
         });
+    if (kStatic && ballot64(valid && walls != 0u) != 0ull && lane == 0u) heavy[blk] = 1u;
     if (!valid) return;
     if (kStatic) walk_walls<kBatchDensity>(walls, cx, cy, cz, S, stable, dWalls);
     const float rho = S.selfDensity + (sumDyn.s * S.mass + sumSt.s * S.mass) * S.C;
@@ -510,6 +565,7 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const Block
                                                      const float4* __restrict__ spos,
                                                      const uint32_t* __restrict__ stab,
                                                      const uint32_t* __restrict__ wallMap,
+                                                     const uint32_t* __restrict__ order,
                                                      const uint32_t* __restrict__ mask,
                                                      const uint32_t* __restrict__ ids, const float* __restrict__ tris,
                                                      float dt, float4* __restrict__ posT, float4* __restrict__ velT,
@@ -517,7 +573,7 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const Block
     __shared__ GroupTables gt;
     const uint32_t tile = xcd_tile(blockIdx.x, numBlocks);
     if (tile >= numBlocks) return;
-    const BlockDesc bd = blocks[tile];
+    const BlockDesc bd = blocks[order[tile]];
     const SceneDev& S = scenes[bd.scene];
     const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
     const uint32_t wbegin = bd.begin + 64u * wave;

@@ -63,6 +63,8 @@ struct sph_engine {
     float4* spos = nullptr;
     uint32_t* stab = nullptr;
     uint32_t* wallMap = nullptr;  // per scene: bit cell_hash(c) set for every cell c holding walls
+    uint32_t* tileOrder = nullptr;  // cell-wave launch order (k_tile_order), gridX entries
+    uint32_t* heavy = nullptr;      // per block: particles near walls in the last density pass
     uint64_t wallWords = 0;
     uint32_t* staticScene = nullptr;
     float* tris = nullptr;
@@ -156,6 +158,8 @@ void release(sph_engine* h) {
     dfree(h->spos);
     dfree(h->stab);
     dfree(h->wallMap);
+    dfree(h->tileOrder);
+    dfree(h->heavy);
     h->wallWords = 0;
     dfree(h->staticScene);
     dfree(h->tris);
@@ -288,10 +292,12 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
 #define SPH_DENS(S, D)                                                                                         \
     if (walk == Walk::CellWave && mixed)                                                                       \
         hipLaunchKernelGGL((k_density_cw<S, D, true>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,   \
-                           h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->wallMap, h->mask, h->dens);     \
+                           h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->wallMap, h->tileOrder, h->heavy, \
+                           h->mask, h->dens);                                                                  \
     else if (walk == Walk::CellWave)                                                                           \
         hipLaunchKernelGGL((k_density_cw<S, D, false>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,  \
-                           h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->wallMap, h->mask, h->dens);     \
+                           h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->wallMap, h->tileOrder, h->heavy, \
+                           h->mask, h->dens);                                                                  \
     else if (walk == Walk::Tiles)                                                                              \
         hipLaunchKernelGGL((k_density_tile<S, D>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,       \
                            h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->mask, h->dens);                 \
@@ -336,12 +342,12 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     do {                                                                                                          \
     if (walk == Walk::CellWave && mixed)                                                                          \
         hipLaunchKernelGGL((k_force_cw<S, D, T, true>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,     \
-                           h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->wallMap, h->mask,          \
+                           h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->wallMap, h->tileOrder, h->mask, \
                            h->ids[c], h->tris,                                                                    \
                            dt, h->posT, h->velT, h->tab[c ^ 1], h->acc);                                          \
     else if (walk == Walk::CellWave)                                                                              \
         hipLaunchKernelGGL((k_force_cw<S, D, T, false>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,    \
-                           h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->wallMap, h->mask,          \
+                           h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->wallMap, h->tileOrder, h->mask, \
                            h->ids[c], h->tris,                                                                    \
                            dt, h->posT, h->velT, h->tab[c ^ 1], h->acc);                                          \
     else if (walk == Walk::Tiles)                                                                                 \
@@ -386,6 +392,11 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     hipLaunchKernelGGL(k_scatter, grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->posT, h->velT, h->ids[c],
                        h->tab[c ^ 1], h->pos, h->vel, h->ids[c ^ 1]);
     SPH_HIP(h, hipGetLastError());
+    // Next step's cell-wave launch order: workgroups near walls first.
+    if (walk == Walk::CellWave && h->hasStatic) {
+        hipLaunchKernelGGL(k_tile_order, dim3(8), dim3(256), 0, h->stream, h->heavy, h->numBlocks, h->tileOrder);
+        SPH_HIP(h, hipGetLastError());
+    }
     if ((rc = end())) return rc;
     return SPH_OK;
 }
@@ -629,6 +640,8 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
     SPH_ALLOC(h->stab, std::max<uint64_t>(h->Ls, 1));
     h->wallWords = hWallMap.size();
     SPH_ALLOC(h->wallMap, std::max<uint64_t>(h->wallWords, 1));
+    SPH_ALLOC(h->tileOrder, std::max<uint64_t>(h->numBlocks, 1));
+    SPH_ALLOC(h->heavy, std::max<uint64_t>(h->numBlocks, 1));
     SPH_ALLOC(h->staticScene, std::max<uint64_t>(Ns, 1));
     SPH_ALLOC(h->consPartial, 8ull * std::max<uint32_t>(h->numBlocks, 1));
     SPH_ALLOC(h->consOut, 8ull * n_scenes);
@@ -659,6 +672,12 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
     SPH_HIP(h, hipMemset(h->status[0], 0, sizeof(unsigned long long) * h->numTiles));
     SPH_HIP(h, hipMemset(h->status[1], 0, sizeof(unsigned long long) * h->numTiles));
     SPH_HIP(h, hipMemset(h->ticket, 0, sizeof(uint32_t)));
+    {
+        std::vector<uint32_t> identity(std::max<uint64_t>(h->numBlocks, 1));
+        for (size_t t = 0; t < identity.size(); ++t) identity[t] = (uint32_t)t;
+        SPH_HIP(h, hipMemcpy(h->tileOrder, identity.data(), sizeof(uint32_t) * identity.size(), hipMemcpyHostToDevice));
+        SPH_HIP(h, hipMemset(h->heavy, 0, sizeof(uint32_t) * identity.size()));
+    }
     if (Ns) {
         SPH_HIP(h, hipMemcpy(h->spos, hSpos.data(), sizeof(float4) * Ns, hipMemcpyHostToDevice));
         SPH_HIP(h, hipMemcpy(h->stab, hStab.data(), sizeof(uint32_t) * h->Ls, hipMemcpyHostToDevice));
