@@ -401,6 +401,7 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
                                                        const uint32_t* __restrict__ wallMap,
                                                        const uint32_t* __restrict__ order,
                                                        uint32_t* __restrict__ heavy,
+                                                       float4* __restrict__ wallSum, float4* __restrict__ wallZ,
                                                        uint32_t* __restrict__ mask, float* __restrict__ dens) {
     __shared__ GroupTables gt;
     const uint32_t tile = xcd_tile(blockIdx.x, numBlocks);
@@ -424,6 +425,12 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
     const uint32_t lo = S.pBegin, hi = S.pBegin + S.pCount;
 
     Acc<kMixed> sumDyn, sumSt;
+    // For the force pass, the walls' geometric sums (SF:590-623 without the
+    // particle's own |p|, which the force pass multiplies in): sum of w1(r) dir,
+    // sum of w2(r), and the w1 of coincident pairs.
+    Acc<kMixed> wgx, wgy, wgz;
+    float wW2 = 0.f, wZero = 0.f;
+    bool wAny = false;
     Mask8 m8 = {{0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u}};  // this lane's neighbour mask, rounds 0..3
     // Mixed precision: this lane's cell corner and its FP16 offset from it, each
     // coordinate twice (both halves), for the candidate filter.
@@ -461,7 +468,19 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
             const float r2 = dx * dx + dy * dy + dz * dz;
             const float r = __builtin_amdgcn_sqrtf(r2);
             const float q = h - r;
-            if (j0 + u < e && r2 <= h2) sumSt.add((r < h) ? q * q * q : 0.f);
+            if (j0 + u < e && r2 <= h2) {
+                const bool in = r < h;
+                sumSt.add(in ? q * q * q : 0.f);
+                const bool zero = r == 0.f;  // r2 == 0 implies r == 0
+                const float inv = zero ? 0.f : __builtin_amdgcn_rcpf(r);
+                const float w1 = in ? -3.0f * q * q : 0.f;
+                wgx.add((dx * inv) * w1);
+                wgy.add((dy * inv) * w1);
+                wgz.add((dz * inv) * w1);
+                wZero += zero ? w1 : 0.f;
+                wAny |= zero;
+                wW2 += in ? 6.0f * q : 0.f;
+            }
         }
     };
     uint32_t walls = 0u;  // static buckets this lane walks
@@ -550,6 +569,12 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
     // Only this lane writes pos[i].w; no lane of this pass reads any .w.
     reinterpret_cast<float*>(pos)[4ull * i + 3] = S.k * (rho - S.rho0);
     if (kDebug) dens[i] = rho;
+    if (kStatic) {
+        // w holds the sum of w2 with its sign bit set when a coincident pair
+        // occurred (the sum itself is never negative); then wallZ holds its w1.
+        wallSum[i] = make_float4(wgx.s, wgy.s, wgz.s, wAny ? -wW2 : wW2);
+        if (wAny) wallZ[i] = make_float4(wZero, 0.f, 0.f, 0.f);
+    }
     uint4* mrow = reinterpret_cast<uint4*>(mask + (size_t)i * kCwMaskWords);
     mrow[0] = make_uint4(m8.w[0], m8.w[1], m8.w[2], m8.w[3]);
     mrow[1] = make_uint4(m8.w[4], m8.w[5], m8.w[6], m8.w[7]);
@@ -564,8 +589,9 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const Block
                                                      const uint32_t* __restrict__ tab,
                                                      const float4* __restrict__ spos,
                                                      const uint32_t* __restrict__ stab,
-                                                     const uint32_t* __restrict__ wallMap,
                                                      const uint32_t* __restrict__ order,
+                                                     const float4* __restrict__ wallSum,
+                                                     const float4* __restrict__ wallZ,
                                                      const uint32_t* __restrict__ mask,
                                                      const uint32_t* __restrict__ ids, const float* __restrict__ tris,
                                                      float dt, float4* __restrict__ posT, float4* __restrict__ velT,
@@ -589,7 +615,6 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const Block
     const uint32_t cx = cell_coord(xi.x, h), cy = cell_coord(xi.y, h), cz = cell_coord(xi.z, h);
     const uint32_t code = morton3(cx, cy, cz);
     const uint32_t* table = tab + S.tBegin;
-    const uint32_t* stable = stab + S.stBegin;
     const uint32_t lo = S.pBegin, hi = S.pBegin + S.pCount;
     uint64_t m[4];
     {
@@ -602,6 +627,7 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const Block
     }
     const float rho = pi / S.k + S.rho0;  // SF:544
     const float absP = fabsf(pi);
+    const float4 ws = kStatic ? ld16(wallSum, iv) : make_float4(0.f, 0.f, 0.f, 0.f);
     Acc<kMixed> fpx, fpy, fpz, fvx, fvy, fvz, fsx, fsy, fsz;
     float sumD2s = 0.f;
     // Pressure weights of coincident pairs (r == 0): their direction is the
@@ -643,34 +669,9 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const Block
 #pragma unroll
         for (uint32_t u = 0; u < kBatchForce; ++u) pair_at(j[u], xj[u], vj[u]);
     };
-    // One static candidate (SF:590-623): |p_i| for both sides, viscosity scaled by 0.
-    auto wall_at = [&](const float4 xj, bool live) {
-        const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        if (!live || r2 > h2) return;
-        const float r = __builtin_amdgcn_sqrtf(r2);
-        const bool zero = r == 0.f;
-        const float inv = zero ? 0.f : __builtin_amdgcn_rcpf(r);
-        const float q = h - r;
-        const bool in = r < h;
-        const float sp = absP * (in ? -3.0f * q * q : 0.f);
-        fsx.add((dx * inv) * sp);
-        fsy.add((dy * inv) * sp);
-        fsz.add((dz * inv) * sp);
-        zeroSp += zero ? sp : 0.f;
-        anyZero |= zero;
-        sumD2s += in ? 6.0f * q : 0.f;
-    };
-    auto walls4 = [&](uint32_t j0, uint32_t e) {
-        float4 xj[4];
-#pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) xj[u] = ld16(spos, j0 + u < e ? j0 + u : j0);
-#pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) wall_at(xj[u], j0 + u < e);
-    };
-    uint32_t walls = 0u;  // static buckets this lane walks
-    for_each_group<kStatic>(
-        lane, wave, valid, code, cx, cy, cz, S, table, wallMap, lo, hi, gt, walls,
+    uint32_t noWalls = 0u;  // walls come from the density pass's sums
+    for_each_group<false>(
+        lane, wave, valid, code, cx, cy, cz, S, table, nullptr, lo, hi, gt, noWalls,
         [&](uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, auto&& between) { between(); },
         [&](uint32_t slot, uint32_t crowded) {
             walk_mask<kBatchForce>(m, gt.start[wave][slot], i, pair4);
@@ -687,7 +688,17 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const Block
             }
         });
     if (!valid) return;
-    if (kStatic) walk_walls<4>(walls, cx, cy, cz, S, stable, walls4);
+    if (kStatic) {
+        // Walls (SF:590-623): |p_i| times the density pass's geometric sums.
+        fsx.add(absP * ws.x);
+        fsy.add(absP * ws.y);
+        fsz.add(absP * ws.z);
+        sumD2s = fabsf(ws.w);
+        if (__builtin_signbit(ws.w)) {
+            zeroSp += absP * wallZ[i].x;
+            anyZero = true;
+        }
+    }
     if (anyZero) {
         const float3 d = random_direction((float)(ids[i] - S.pBegin));
         fpx.add(d.x * zeroSp);

@@ -65,6 +65,8 @@ struct sph_engine {
     uint32_t* wallMap = nullptr;  // per scene: bit cell_hash(c) set for every cell c holding walls
     uint32_t* tileOrder = nullptr;  // cell-wave launch order (k_tile_order), gridX entries
     uint32_t* heavy = nullptr;      // per block: particles near walls in the last density pass
+    float4* wallSum = nullptr;      // cell-wave: the walls' geometric sums of each particle, for the force pass
+    float4* wallZ = nullptr;        // and the coincident-pair weight where one occurred
     uint64_t wallWords = 0;
     uint32_t* staticScene = nullptr;
     float* tris = nullptr;
@@ -160,6 +162,8 @@ void release(sph_engine* h) {
     dfree(h->wallMap);
     dfree(h->tileOrder);
     dfree(h->heavy);
+    dfree(h->wallSum);
+    dfree(h->wallZ);
     h->wallWords = 0;
     dfree(h->staticScene);
     dfree(h->tris);
@@ -291,13 +295,13 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     if (walk != Walk::Cells) {
 #define SPH_DENS(S, D)                                                                                         \
     if (walk == Walk::CellWave && mixed)                                                                       \
-        hipLaunchKernelGGL((k_density_cw<S, D, true>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,   \
-                           h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->wallMap, h->tileOrder, h->heavy, \
-                           h->mask, h->dens);                                                                  \
+        hipLaunchKernelGGL((k_density_cw<S, D, true>), gridX, block, 0, h->stream, h->dBlocks,            \
+                           h->numBlocks, h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->wallMap,        \
+                           h->tileOrder, h->heavy, h->wallSum, h->wallZ, h->mask, h->dens);                   \
     else if (walk == Walk::CellWave)                                                                           \
-        hipLaunchKernelGGL((k_density_cw<S, D, false>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,  \
-                           h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->wallMap, h->tileOrder, h->heavy, \
-                           h->mask, h->dens);                                                                  \
+        hipLaunchKernelGGL((k_density_cw<S, D, false>), gridX, block, 0, h->stream, h->dBlocks,            \
+                           h->numBlocks, h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->wallMap,        \
+                           h->tileOrder, h->heavy, h->wallSum, h->wallZ, h->mask, h->dens);                   \
     else if (walk == Walk::Tiles)                                                                              \
         hipLaunchKernelGGL((k_density_tile<S, D>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,       \
                            h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->mask, h->dens);                 \
@@ -341,15 +345,15 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
 #define SPH_FORCE(S, D, T)                                                                                        \
     do {                                                                                                          \
     if (walk == Walk::CellWave && mixed)                                                                          \
-        hipLaunchKernelGGL((k_force_cw<S, D, T, true>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,     \
-                           h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->wallMap, h->tileOrder, h->mask, \
-                           h->ids[c], h->tris,                                                                    \
-                           dt, h->posT, h->velT, h->tab[c ^ 1], h->acc);                                          \
+        hipLaunchKernelGGL((k_force_cw<S, D, T, true>), gridX, block, 0, h->stream, h->dBlocks,                \
+                           h->numBlocks, h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->tileOrder,  \
+                           h->wallSum, h->wallZ, h->mask, h->ids[c], h->tris, dt, h->posT, h->velT,              \
+                           h->tab[c ^ 1], h->acc);                                                                \
     else if (walk == Walk::CellWave)                                                                              \
-        hipLaunchKernelGGL((k_force_cw<S, D, T, false>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,    \
-                           h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->wallMap, h->tileOrder, h->mask, \
-                           h->ids[c], h->tris,                                                                    \
-                           dt, h->posT, h->velT, h->tab[c ^ 1], h->acc);                                          \
+        hipLaunchKernelGGL((k_force_cw<S, D, T, false>), gridX, block, 0, h->stream, h->dBlocks,                \
+                           h->numBlocks, h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->tileOrder,  \
+                           h->wallSum, h->wallZ, h->mask, h->ids[c], h->tris, dt, h->posT, h->velT,              \
+                           h->tab[c ^ 1], h->acc);                                                                \
     else if (walk == Walk::Tiles)                                                                                 \
         hipLaunchKernelGGL((k_force_tile<S, D, T>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,         \
                            h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->mask, h->ids[c], h->tris,  \
@@ -642,6 +646,10 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
     SPH_ALLOC(h->wallMap, std::max<uint64_t>(h->wallWords, 1));
     SPH_ALLOC(h->tileOrder, std::max<uint64_t>(h->numBlocks, 1));
     SPH_ALLOC(h->heavy, std::max<uint64_t>(h->numBlocks, 1));
+    if (Ns) {
+        SPH_ALLOC(h->wallSum, std::max<uint64_t>(N, 1));
+        SPH_ALLOC(h->wallZ, std::max<uint64_t>(N, 1));
+    }
     SPH_ALLOC(h->staticScene, std::max<uint64_t>(Ns, 1));
     SPH_ALLOC(h->consPartial, 8ull * std::max<uint32_t>(h->numBlocks, 1));
     SPH_ALLOC(h->consOut, 8ull * n_scenes);
