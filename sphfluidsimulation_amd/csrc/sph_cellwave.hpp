@@ -104,7 +104,15 @@ __device__ inline void settle(uint32_t (&j)[8]) {
 #define SPH_CW_BATCH_FORCE 2
 #endif
 constexpr uint32_t kBatchDensity = SPH_CW_BATCH_DENSITY;
+// Density pass, crowded buckets' extra particles.
+#ifndef SPH_CW_BATCH_CROWDED
+#define SPH_CW_BATCH_CROWDED 2
+#endif
+constexpr uint32_t kBatchCrowded = SPH_CW_BATCH_CROWDED;
 constexpr uint32_t kBatchForce = SPH_CW_BATCH_FORCE;
+// Crowded buckets: the density pass records which of a particle's first
+// kExtraBits extra candidates are neighbours, for the force pass.
+constexpr uint32_t kExtraBits = 32;
 
 // Walks the set bits of a neighbour mask B at a time: f(j) gets B candidate
 // indices (bucket start of the bit's bucket + slot), `self` standing in for
@@ -402,7 +410,8 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
                                                        const uint32_t* __restrict__ order,
                                                        uint32_t* __restrict__ heavy,
                                                        float4* __restrict__ wallSum, float4* __restrict__ wallZ,
-                                                       uint32_t* __restrict__ mask, float* __restrict__ dens) {
+                                                       uint32_t* __restrict__ mask,
+                                                       uint32_t* __restrict__ extraMask, float* __restrict__ dens) {
     __shared__ GroupTables gt;
     const uint32_t tile = xcd_tile(blockIdx.x, numBlocks);
     if (tile >= numBlocks) return;
@@ -417,7 +426,7 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
     const bool valid = lane < wcount;
     const float4 xi = ld16(pos, valid ? i : wbegin);
     const float h = S.h, h2 = S.h2;
-    const float h2wide = h2 * 1.0001f;
+    const float h2wide = S.h2wide;
     const uint32_t cx = cell_coord(xi.x, h), cy = cell_coord(xi.y, h), cz = cell_coord(xi.z, h);
     const uint32_t code = morton3(cx, cy, cz);
     const uint32_t* table = tab + S.tBegin;
@@ -444,19 +453,26 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
         tuz = h2_bits(pack_h2(uz, uz));
     }
     // Exact contributions of four candidates j (SF:426-453).
-    auto d4 = [&](const uint32_t (&j)[kBatchDensity]) {
-        float4 xj[kBatchDensity];
+    // Returns bit u set for each candidate within h (r2 <= h2, j != i).
+    auto d4 = [&](const auto& j) -> uint32_t {
+        constexpr uint32_t B = sizeof(j) / sizeof(j[0]);
+        float4 xj[B];
 #pragma unroll
-        for (uint32_t u = 0; u < kBatchDensity; ++u) xj[u] = ld16(pos, j[u]);
+        for (uint32_t u = 0; u < B; ++u) xj[u] = ld16(pos, j[u]);
+        uint32_t hits = 0u;
 #pragma unroll
-        for (uint32_t u = 0; u < kBatchDensity; ++u) {
+        for (uint32_t u = 0; u < B; ++u) {
             const float dx = xj[u].x - xi.x, dy = xj[u].y - xi.y, dz = xj[u].z - xi.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
             const float r = __builtin_amdgcn_sqrtf(r2);
             const float q = h - r;
-            sumDyn.add((r2 <= h2 && j[u] != i && r < h) ? q * q * q : 0.f);
+            const bool ok = r2 <= h2 && j[u] != i;
+            sumDyn.add((ok && r < h) ? q * q * q : 0.f);
+            hits |= ok ? 1u << u : 0u;
         }
+        return hits;
     };
+
     // Static neighbours (SF:455-496).
     auto dWalls = [&](uint32_t j0, uint32_t e) {
         float4 xj[kBatchDensity];
@@ -551,19 +567,30 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
             for (uint32_t r = 0; r < 4; ++r) m[r] = ((uint64_t)m8.w[2 * r + 1] << 32) | m8.w[2 * r];
             walk_mask<kBatchDensity>(m, gt.start[wave][slot], i, d4);
             // Crowded buckets: their particles past the kSlots-th, a batch at a time.
+            // extraMask[i] bit k: extra particle k, enumerated bucket by bucket in
+            // visit order, is a neighbour; all ones when there are more than
+            // kExtraBits extras. The force pass reads it.
+            if (!crowded) return;
+            uint32_t k = 0, bits = 0u;
             for (; crowded; crowded &= crowded - 1u) {
                 const uint32_t bi = (uint32_t)__builtin_ctz(crowded);
-                const uint32_t s0 = gt.start[wave][slot][bi], n = gt.cnt[wave][slot][bi];
-                for (uint32_t t = kSlots; t < n; t += kBatchDensity) {
-                    uint32_t j[kBatchDensity];
+                const uint32_t s0 = gt.start[wave][slot][bi], e = s0 + gt.cnt[wave][slot][bi];
+                for (uint32_t jj = s0 + kSlots; jj < e; jj += kBatchCrowded) {
+                    uint32_t j[kBatchCrowded];
 #pragma unroll
-                    for (uint32_t u = 0; u < kBatchDensity; ++u) j[u] = t + u < n ? s0 + t + u : i;
-                    d4(j);
+                    for (uint32_t u = 0; u < kBatchCrowded; ++u) j[u] = jj + u < e ? jj + u : i;
+                    const uint32_t hits = d4(j);  // padded entries (j = i) never hit
+                    if (k < kExtraBits) bits |= hits << k;
+                    k += min(kBatchCrowded, e - jj);
                 }
             }
+            extraMask[i] = k > kExtraBits ? ~0u : bits;
         });
     if (kStatic && ballot64(valid && walls != 0u) != 0ull && lane == 0u) heavy[blk] = 1u;
     if (!valid) return;
+    uint4* mrow = reinterpret_cast<uint4*>(mask + (size_t)i * kCwMaskWords);
+    mrow[0] = make_uint4(m8.w[0], m8.w[1], m8.w[2], m8.w[3]);
+    mrow[1] = make_uint4(m8.w[4], m8.w[5], m8.w[6], m8.w[7]);
     if (kStatic) walk_walls<kBatchDensity>(walls, cx, cy, cz, S, stable, dWalls);
     const float rho = S.selfDensity + (sumDyn.s * S.mass + sumSt.s * S.mass) * S.C;
     // Only this lane writes pos[i].w; no lane of this pass reads any .w.
@@ -575,9 +602,6 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
         wallSum[i] = make_float4(wgx.s, wgy.s, wgz.s, wAny ? -wW2 : wW2);
         if (wAny) wallZ[i] = make_float4(wZero, 0.f, 0.f, 0.f);
     }
-    uint4* mrow = reinterpret_cast<uint4*>(mask + (size_t)i * kCwMaskWords);
-    mrow[0] = make_uint4(m8.w[0], m8.w[1], m8.w[2], m8.w[3]);
-    mrow[1] = make_uint4(m8.w[4], m8.w[5], m8.w[6], m8.w[7]);
 }
 
 // Eight waves per SIMD (at most 64 VGPRs) in FP32; the Kahan sums of the mixed
@@ -593,6 +617,7 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const Block
                                                      const float4* __restrict__ wallSum,
                                                      const float4* __restrict__ wallZ,
                                                      const uint32_t* __restrict__ mask,
+                                                     const uint32_t* __restrict__ extraMask,
                                                      const uint32_t* __restrict__ ids, const float* __restrict__ tris,
                                                      float dt, float4* __restrict__ posT, float4* __restrict__ velT,
                                                      uint32_t* __restrict__ counts, float4* __restrict__ accOut) {
@@ -627,7 +652,6 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const Block
     }
     const float rho = pi / S.k + S.rho0;  // SF:544
     const float absP = fabsf(pi);
-    const float4 ws = kStatic ? ld16(wallSum, iv) : make_float4(0.f, 0.f, 0.f, 0.f);
     Acc<kMixed> fpx, fpy, fpz, fvx, fvy, fvz, fsx, fsy, fsz;
     float sumD2s = 0.f;
     // Pressure weights of coincident pairs (r == 0): their direction is the
@@ -675,21 +699,36 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const Block
         [&](uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, auto&& between) { between(); },
         [&](uint32_t slot, uint32_t crowded) {
             walk_mask<kBatchForce>(m, gt.start[wave][slot], i, pair4);
-            // Crowded buckets: their particles past the kSlots-th, a batch at a time.
+            // Crowded buckets' extra particles: the density pass's neighbours
+            // among them when there are at most kExtraBits, else every one, a
+            // batch at a time.
+            if (!crowded) return;
+            uint32_t xb = extraMask[i];
+            const bool all = xb == ~0u;
             for (; crowded; crowded &= crowded - 1u) {
                 const uint32_t bi = (uint32_t)__builtin_ctz(crowded);
-                const uint32_t s0 = gt.start[wave][slot][bi], n = gt.cnt[wave][slot][bi];
-                for (uint32_t t = kSlots; t < n; t += kBatchForce) {
-                    uint32_t j[kBatchForce];
+                const uint32_t s0 = gt.start[wave][slot][bi] + kSlots, ne = gt.cnt[wave][slot][bi] - kSlots;
+                for (uint32_t base = 0; base < ne; base += 32u) {
+                    const uint32_t c = min(32u, ne - base);
+                    const uint32_t low = c < 32u ? (1u << c) - 1u : ~0u;
+                    uint32_t w = all ? low : xb & low;
+                    xb = c < 32u ? xb >> c : 0u;
+                    while (w) {
+                        uint32_t j[kBatchForce];
 #pragma unroll
-                    for (uint32_t u = 0; u < kBatchForce; ++u) j[u] = t + u < n ? s0 + t + u : i;
-                    pair4(j);
+                        for (uint32_t u = 0; u < kBatchForce; ++u) {
+                            j[u] = w ? s0 + base + (uint32_t)__builtin_ctz(w) : i;
+                            w &= w - 1u;
+                        }
+                        pair4(j);
+                    }
                 }
             }
         });
     if (!valid) return;
     if (kStatic) {
         // Walls (SF:590-623): |p_i| times the density pass's geometric sums.
+        const float4 ws = ld16(wallSum, i);
         fsx.add(absP * ws.x);
         fsy.add(absP * ws.y);
         fsz.add(absP * ws.z);

@@ -72,7 +72,8 @@ struct sph_engine {
     float* tris = nullptr;
     float* dens = nullptr;
     float4* acc = nullptr;
-    uint32_t* mask = nullptr;  // candidate mask from the density pass (run walk)
+    uint32_t* mask = nullptr;       // candidate mask from the density pass (run, tile and cell-wave walks)
+    uint32_t* extraMask = nullptr;  // cell-wave: neighbours among crowded buckets' extra particles
     double* consPartial = nullptr;
     double* consOut = nullptr;
     uint32_t* valBits = nullptr;
@@ -170,6 +171,7 @@ void release(sph_engine* h) {
     dfree(h->dens);
     dfree(h->acc);
     dfree(h->mask);
+    dfree(h->extraMask);
     dfree(h->consPartial);
     dfree(h->consOut);
     dfree(h->valBits);
@@ -297,11 +299,11 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     if (walk == Walk::CellWave && mixed)                                                                       \
         hipLaunchKernelGGL((k_density_cw<S, D, true>), gridX, block, 0, h->stream, h->dBlocks,            \
                            h->numBlocks, h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->wallMap,        \
-                           h->tileOrder, h->heavy, h->wallSum, h->wallZ, h->mask, h->dens);                   \
+                           h->tileOrder, h->heavy, h->wallSum, h->wallZ, h->mask, h->extraMask, h->dens);     \
     else if (walk == Walk::CellWave)                                                                           \
         hipLaunchKernelGGL((k_density_cw<S, D, false>), gridX, block, 0, h->stream, h->dBlocks,            \
                            h->numBlocks, h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->wallMap,        \
-                           h->tileOrder, h->heavy, h->wallSum, h->wallZ, h->mask, h->dens);                   \
+                           h->tileOrder, h->heavy, h->wallSum, h->wallZ, h->mask, h->extraMask, h->dens);     \
     else if (walk == Walk::Tiles)                                                                              \
         hipLaunchKernelGGL((k_density_tile<S, D>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,       \
                            h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->mask, h->dens);                 \
@@ -347,13 +349,13 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     if (walk == Walk::CellWave && mixed)                                                                          \
         hipLaunchKernelGGL((k_force_cw<S, D, T, true>), gridX, block, 0, h->stream, h->dBlocks,                \
                            h->numBlocks, h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->tileOrder,  \
-                           h->wallSum, h->wallZ, h->mask, h->ids[c], h->tris, dt, h->posT, h->velT,              \
-                           h->tab[c ^ 1], h->acc);                                                                \
+                           h->wallSum, h->wallZ, h->mask, h->extraMask, h->ids[c], h->tris, dt, h->posT,         \
+                           h->velT, h->tab[c ^ 1], h->acc);                                                       \
     else if (walk == Walk::CellWave)                                                                              \
         hipLaunchKernelGGL((k_force_cw<S, D, T, false>), gridX, block, 0, h->stream, h->dBlocks,                \
                            h->numBlocks, h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->tileOrder,  \
-                           h->wallSum, h->wallZ, h->mask, h->ids[c], h->tris, dt, h->posT, h->velT,              \
-                           h->tab[c ^ 1], h->acc);                                                                \
+                           h->wallSum, h->wallZ, h->mask, h->extraMask, h->ids[c], h->tris, dt, h->posT,         \
+                           h->velT, h->tab[c ^ 1], h->acc);                                                       \
     else if (walk == Walk::Tiles)                                                                                 \
         hipLaunchKernelGGL((k_force_tile<S, D, T>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,         \
                            h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->mask, h->ids[c], h->tris,  \
@@ -544,6 +546,7 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
         S.gz = P.gravityZ;
         S.C = kernel_constant(S.h);                     // SimulationEngineCPU.cpp:406
         S.selfDensity = S.mass * d0_at_zero(S.h) * S.C;  // SimulationEngineCPU.cpp:407
+        S.h2wide = S.h2 * 1.0001f;
         const uint64_t p0 = dyn_offsets[s], pn = dyn_offsets[s + 1] - p0;
         const uint64_t s0 = static_offsets[s], sn = static_offsets[s + 1] - s0;
         S.pBegin = (uint32_t)p0;
@@ -657,7 +660,10 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
     SPH_ALLOC(h->valIdx, 8ull * n_scenes);
     static_assert(kMaskWords == kTileMaskWords && kMaskWords == kCwMaskWords,
                   "every mask format uses 8 words per particle");
-    if (!(flags & SPH_FLAG_CELL_WALK)) SPH_ALLOC(h->mask, kMaskWords * std::max<uint64_t>(N, 1));  // tile and run walks
+    if (!(flags & SPH_FLAG_CELL_WALK)) {
+        SPH_ALLOC(h->mask, kMaskWords * std::max<uint64_t>(N, 1));
+        SPH_ALLOC(h->extraMask, std::max<uint64_t>(N, 1));
+    }
     if (flags & SPH_FLAG_RECORD_DEBUG) {
         SPH_ALLOC(h->dens, std::max<uint64_t>(N, 1));
         SPH_ALLOC(h->acc, std::max<uint64_t>(N, 1));

@@ -29,7 +29,7 @@ constexpr int kBlock = 256;  // particles per workgroup in the particle kernels
 struct alignas(16) SceneDev {
     float h, h2, mass, k;
     float rho0, mu, gx, gy;
-    float gz, selfDensity, C, pad0;
+    float gz, selfDensity, C, h2wide;  // h2wide = h2 * 1.0001f, the candidate filters' radius
     uint32_t pBegin, pCount, tBegin, M;  // particles [pBegin, pBegin + pCount), table [tBegin, tBegin + M]
     uint64_t magicM;
     uint32_t sBegin, sCount;              // static particles [sBegin, sBegin + sCount)
@@ -378,7 +378,7 @@ __global__ __launch_bounds__(kBlock) void k_density_runs(const BlockDesc* __rest
     for (uint32_t w = 0; w < kMaskWords; ++w) smask[w * kBlock + tid] = 0u;
     const float4 xi = pos[i];
     const float h = S.h, h2 = S.h2;
-    const float h2wide = h2 * 1.0001f;
+    const float h2wide = S.h2wide;
     const uint32_t cx = cell_coord(xi.x, h), cy = cell_coord(xi.y, h), cz = cell_coord(xi.z, h);
     float sumDyn = 0.f, sumSt = 0.f;
     uint32_t k = 0, bits = 0;

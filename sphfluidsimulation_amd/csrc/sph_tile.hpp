@@ -191,7 +191,7 @@ __global__ __launch_bounds__(kBlock) void k_density_tile(const BlockDesc* __rest
     const bool valid = i < bd.end;
     const float4 xi = valid ? pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float h = S.h, h2 = S.h2;
-    const float h2wide = h2 * 1.0001f;
+    const float h2wide = S.h2wide;
     const uint32_t cx = cell_coord(xi.x, h), cy = cell_coord(xi.y, h), cz = cell_coord(xi.z, h);
     const CellSpread cs = cell_spread(xi.x, xi.y, xi.z, h);
     const uint32_t code = morton3(cx, cy, cz);
