@@ -205,7 +205,12 @@ __device__ inline RoundCand round_candidate(uint32_t r, uint32_t lane, uint32_t 
 struct GroupFetch {
     uint32_t g0, g1, tb, te;  // lanes g0..g1-1; dynamic table entries of neighbour cell `lane`
     uint32_t wword;           // the word of the wall-cell bitmap holding that cell's bit
+    bool poison;              // the group's particles visit a bucket holding a NaN particle
+    bool nanGroup;            // one of the group's own particles has a NaN coordinate
 };
+
+// Bit of myWalls (for_each_group) that marks a poisoned particle.
+constexpr uint32_t kPoisonBit = 0x80000000u;
 
 // Neighbour cell `lane` (lanes past 26 repeat cell 26) of the group whose first
 // lane is g0, in the reference's x-y-z visit order (SF:397-404).
@@ -220,10 +225,21 @@ __device__ inline void neighbour_cell(uint32_t lane, uint32_t g0, uint32_t cx, u
     nz = gz + n % 3u - 1u;
 }
 
+// NaN particles. A particle with a NaN coordinate has the NaN distance to every
+// candidate, which the reference keeps (within_h2), so every particle that
+// visits its bucket ends the step with NaN density, pressure, velocity and
+// position. Fully NaN positions all fall in cell 0, bucket 0 (cell_coord of NaN
+// is 0), where they pile up. `nan0` counts the particles with a NaN coordinate
+// in bucket 0 of the scene (the force pass counts them for the next step):
+// when a group visits bucket 0 and one of them is not the group's own particle
+// (nan0 >= 2, or no particle of the group has a NaN coordinate), every particle
+// of the group is poisoned, so the walk drops bucket 0 (its range reads as
+// empty) instead of walking the pile, and the passes write NaN results.
+// NaN particles outside bucket 0 are walked like any other.
 template <bool kStatic>
 __device__ inline GroupFetch fetch_group(uint64_t& heads, uint64_t stops, uint32_t lane, uint32_t cx, uint32_t cy,
                                          uint32_t cz, const SceneDev& S, const uint32_t* __restrict__ table,
-                                         const uint32_t* __restrict__ wallMap) {
+                                         const uint32_t* __restrict__ wallMap, uint32_t nan0, uint64_t nanLanes) {
     GroupFetch f;
     f.g0 = (uint32_t)__builtin_ctzll(heads);
     heads &= heads - 1ull;
@@ -231,10 +247,14 @@ __device__ inline GroupFetch fetch_group(uint64_t& heads, uint64_t stops, uint32
     f.g1 = above ? (uint32_t)__builtin_ctzll(above) : 64u;
     uint32_t nx, ny, nz;
     neighbour_cell(lane, f.g0, cx, cy, cz, nx, ny, nz);
+    const uint32_t bk = fastmod(morton3(nx, ny, nz), S.magicM, S.M);
+    const uint64_t groupLanes = (f.g1 < 64u ? (1ull << f.g1) : 0ull) - (1ull << f.g0);
+    f.nanGroup = (nanLanes & groupLanes) != 0ull;
+    f.poison = nan0 != 0u && (nan0 >= 2u || !f.nanGroup) && ballot64(lane < 27u && bk == 0u) != 0ull;
     // Every lane reads (lanes past 26 repeat cell 26; the reader ignores them), so
     // the reads are not in a branch and a wait for later loads need not cover them.
-    const uint32_t bk = fastmod(morton3(nx, ny, nz), S.magicM, S.M);
-    f.tb = table[bk];
+    // A dropped bucket 0 reads entry 1 twice: an empty range.
+    f.tb = table[f.poison && bk == 0u ? 1u : bk];
     f.te = table[bk + 1];
     f.wword = kStatic ? wallMap[S.wmBegin + ((cell_hash(nx, ny, nz) >> 5) & S.wmMask)] : 0u;
     return f;
@@ -247,13 +267,15 @@ __device__ inline GroupFetch fetch_group(uint64_t& heads, uint64_t stops, uint32
 // the reference walks a bucket once per neighbour cell mapping to it
 // (SF:455-496). A skipped bucket holds only walls of cells outside the 27,
 // farther than h, which the reference tests and drops: skipping them changes no
-// sum.
+// sum. The exception is a particle with a NaN coordinate, whose distances are
+// NaN and kept; a group holding one (`all`) walks the static buckets of all 27
+// cells.
 __device__ inline uint32_t wall_visits(uint32_t wword, uint32_t lane, uint32_t g0, uint32_t cx, uint32_t cy,
-                                       uint32_t cz, const SceneDev& S) {
+                                       uint32_t cz, const SceneDev& S, bool all) {
     if (S.sCount == 0u) return 0u;
     uint32_t nx, ny, nz;
     neighbour_cell(lane, g0, cx, cy, cz, nx, ny, nz);
-    const bool hit = lane < 27u && ((wword >> (cell_hash(nx, ny, nz) & 31u)) & 1u);
+    const bool hit = lane < 27u && (all || ((wword >> (cell_hash(nx, ny, nz) & 31u)) & 1u));
     uint32_t hits = (uint32_t)ballot64(hit);
     if (hits == 0u) return 0u;
     const uint32_t sb = fastmod(morton3(nx, ny, nz), S.magicMs, S.Ms);
@@ -274,18 +296,21 @@ __device__ inline uint32_t wall_visits(uint32_t wword, uint32_t lane, uint32_t g
 // overlap the group's work. Then phase2(slot, crowded) runs on the lanes of the
 // batch's groups with `slot` their group's table slot and `crowded` the bit set
 // of its buckets holding more than kSlots particles. myWalls receives, on every
-// lane of a group, the static buckets its particles walk (wall_visits).
+// lane of a group, the static buckets its particles walk (wall_visits), or
+// kPoisonBit alone for a poisoned group (fetch_group), which walks no walls.
+// nanLanes: the lanes whose particle has a NaN coordinate.
 template <bool kStatic, class P1, class P2>
 __device__ inline void for_each_group(uint32_t lane, uint32_t wave, bool valid, uint32_t code, uint32_t cx,
                                       uint32_t cy, uint32_t cz, const SceneDev& S, const uint32_t* __restrict__ table,
-                                      const uint32_t* __restrict__ wallMap, uint32_t lo, uint32_t hi, GroupTables& gt,
-                                      uint32_t& myWalls, P1&& phase1, P2&& phase2) {
+                                      const uint32_t* __restrict__ wallMap, uint32_t nan0, uint64_t nanLanes,
+                                      uint32_t lo, uint32_t hi, GroupTables& gt, uint32_t& myWalls, P1&& phase1,
+                                      P2&& phase2) {
     const uint32_t prevCode = (uint32_t)__shfl_up((int)code, 1);
     uint64_t heads = ballot64(valid && (lane == 0u || prevCode != code));
     const uint64_t stops = heads | ~ballot64(valid);  // a group ends at the next head or invalid lane
     while (heads) {
         uint32_t mySlot = 0xFFFFFFFFu, myCrowded = 0u;
-        GroupFetch cur = fetch_group<kStatic>(heads, stops, lane, cx, cy, cz, S, table, wallMap);
+        GroupFetch cur = fetch_group<kStatic>(heads, stops, lane, cx, cy, cz, S, table, wallMap, nan0, nanLanes);
         for (uint32_t slot = 0;; ++slot) {
             const uint32_t s0 = cur.tb > lo ? cur.tb : lo, e0 = cur.te < hi ? cur.te : hi;
             const uint32_t start27 = s0, cnt27 = lane < 27u && e0 > s0 ? e0 - s0 : 0u;  // empty past lane 26
@@ -294,17 +319,20 @@ __device__ inline void for_each_group(uint32_t lane, uint32_t wave, bool valid, 
                 gt.cnt[wave][slot][lane] = cnt27;
             }
             const uint32_t crowded = (uint32_t)ballot64(cnt27 > kSlots);  // buckets 0..26
-            const uint32_t walls = kStatic ? wall_visits(cur.wword, lane, cur.g0, cx, cy, cz, S) : 0u;
+            const uint32_t walls =
+                kStatic && !cur.poison ? wall_visits(cur.wword, lane, cur.g0, cx, cy, cz, S, cur.nanGroup) : 0u;
             if (lane >= cur.g0 && lane < cur.g1) {
                 mySlot = slot;
                 myCrowded = crowded;
-                myWalls = walls;
+                myWalls = cur.poison ? kPoisonBit : walls;
             }
             const bool more = heads != 0ull && slot + 1u < kGroupBatch;
             GroupFetch nxt;
             uint64_t rest = more ? heads : 1ull;  // at the batch end: a harmless read of group 0's table
             phase1(slot, cur.g0, cur.g1, start27, cnt27,
-                   [&]() { nxt = fetch_group<kStatic>(rest, stops, lane, cx, cy, cz, S, table, wallMap); });
+                   [&]() {
+                       nxt = fetch_group<kStatic>(rest, stops, lane, cx, cy, cz, S, table, wallMap, nan0, nanLanes);
+                   });
             if (!more) break;
             heads = rest;
             cur = nxt;
@@ -411,7 +439,8 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
                                                        uint32_t* __restrict__ heavy,
                                                        float4* __restrict__ wallSum, float4* __restrict__ wallZ,
                                                        uint32_t* __restrict__ mask,
-                                                       uint32_t* __restrict__ extraMask, float* __restrict__ dens) {
+                                                       uint32_t* __restrict__ extraMask,
+                                                       const uint32_t* __restrict__ nanCnt, float* __restrict__ dens) {
     __shared__ GroupTables gt;
     const uint32_t tile = xcd_tile(blockIdx.x, numBlocks);
     if (tile >= numBlocks) return;
@@ -466,8 +495,8 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
             const float r2 = dx * dx + dy * dy + dz * dz;
             const float r = __builtin_amdgcn_sqrtf(r2);
             const float q = h - r;
-            const bool ok = r2 <= h2 && j[u] != i;
-            sumDyn.add((ok && r < h) ? q * q * q : 0.f);
+            const bool ok = within_h2(r2, h2) && j[u] != i;
+            sumDyn.add((ok && inside_h(r, h)) ? q * q * q : 0.f);
             hits |= ok ? 1u << u : 0u;
         }
         return hits;
@@ -484,8 +513,8 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
             const float r2 = dx * dx + dy * dy + dz * dz;
             const float r = __builtin_amdgcn_sqrtf(r2);
             const float q = h - r;
-            if (j0 + u < e && r2 <= h2) {
-                const bool in = r < h;
+            if (j0 + u < e && within_h2(r2, h2)) {
+                const bool in = inside_h(r, h);
                 sumSt.add(in ? q * q * q : 0.f);
                 const bool zero = r == 0.f;  // r2 == 0 implies r == 0
                 const float inv = zero ? 0.f : __builtin_amdgcn_rcpf(r);
@@ -499,9 +528,10 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
             }
         }
     };
-    uint32_t walls = 0u;  // static buckets this lane walks
+    uint32_t walls = 0u;  // static buckets this lane walks, or kPoisonBit
+    const uint64_t nanLanes = ballot64(valid && (xi.x != xi.x || xi.y != xi.y || xi.z != xi.z));
     for_each_group<kStatic>(
-        lane, wave, valid, code, cx, cy, cz, S, table, wallMap, lo, hi, gt, walls,
+        lane, wave, valid, code, cx, cy, cz, S, table, wallMap, nanCnt[bd.scene], nanLanes, lo, hi, gt, walls,
         [&](uint32_t, uint32_t g0, uint32_t g1, uint32_t start27, uint32_t cnt27, auto&& between) {
             // The particle itself passes its own test (r = 0); d4 skips it. Empty
             // slots are masked out with the (wave-uniform) slot masks.
@@ -542,8 +572,8 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
                         for (uint32_t p = 0; p < 2; ++p) {
                             const half2v dx = X[p] - TX, dy = Y[p] - TY, dz = Z[p] - TZ;
                             const half2v d2 = dx * dx + dy * dy + dz * dz;
-                            b[2 * p] = ballot64(d2.x <= R) & occupied[2 * p];
-                            b[2 * p + 1] = ballot64(d2.y <= R) & occupied[2 * p + 1];
+                            b[2 * p] = ballot64(!(d2.x > R)) & occupied[2 * p];  // NaN passes
+                            b[2 * p + 1] = ballot64(!(d2.y > R)) & occupied[2 * p + 1];
                         }
                         writelane4(m8, b, t);
                     }
@@ -556,7 +586,9 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
 #pragma unroll
                 for (uint32_t r = 0; r < 4; ++r) {
                     const float dx = xc[r].x - px, dy = xc[r].y - py, dz = xc[r].z - pz;
-                    b[r] = ballot64(dx * dx + dy * dy + dz * dz <= h2wide) & occupied[r];
+                    b[r] = ballot64(within_h2(dx * dx + dy * dy + dz * dz, h2wide)) & occupied[r];
+                    // Two rounds at a time: fewer temporaries live at once.
+                    if (r == 1u) __builtin_amdgcn_sched_barrier(0);
                 }
                 writelane4(m8, b, t);
             }
@@ -586,13 +618,15 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
             }
             extraMask[i] = k > kExtraBits ? ~0u : bits;
         });
-    if (kStatic && ballot64(valid && walls != 0u) != 0ull && lane == 0u) heavy[blk] = 1u;
+    if (kStatic && ballot64(valid && (walls & ~kPoisonBit) != 0u) != 0ull && lane == 0u) heavy[blk] = 1u;
     if (!valid) return;
     uint4* mrow = reinterpret_cast<uint4*>(mask + (size_t)i * kCwMaskWords);
     mrow[0] = make_uint4(m8.w[0], m8.w[1], m8.w[2], m8.w[3]);
     mrow[1] = make_uint4(m8.w[4], m8.w[5], m8.w[6], m8.w[7]);
-    if (kStatic) walk_walls<kBatchDensity>(walls, cx, cy, cz, S, stable, dWalls);
-    const float rho = S.selfDensity + (sumDyn.s * S.mass + sumSt.s * S.mass) * S.C;
+    if (kStatic) walk_walls<kBatchDensity>(walls & ~kPoisonBit, cx, cy, cz, S, stable, dWalls);
+    // A poisoned particle's density is NaN, as the pile it skipped makes it.
+    const float rho = (walls & kPoisonBit) ? __builtin_nanf("")
+                                           : S.selfDensity + (sumDyn.s * S.mass + sumSt.s * S.mass) * S.C;
     // Only this lane writes pos[i].w; no lane of this pass reads any .w.
     reinterpret_cast<float*>(pos)[4ull * i + 3] = S.k * (rho - S.rho0);
     if (kDebug) dens[i] = rho;
@@ -618,6 +652,7 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const Block
                                                      const float4* __restrict__ wallZ,
                                                      const uint32_t* __restrict__ mask,
                                                      const uint32_t* __restrict__ extraMask,
+                                                     const uint32_t* __restrict__ nanCnt, uint32_t* __restrict__ nanNext,
                                                      const uint32_t* __restrict__ ids, const float* __restrict__ tris,
                                                      float dt, float4* __restrict__ posT, float4* __restrict__ velT,
                                                      uint32_t* __restrict__ counts, float4* __restrict__ accOut) {
@@ -660,20 +695,22 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const Block
     float zeroSp = 0.f;
     bool anyZero = false;
     // One dynamic candidate j (SF:557-589), with its position (pressure in w)
-    // and velocity already loaded. Candidates out of range (or j == i) get zero
-    // kernel weights, so they add exact zeros.
+    // and velocity already loaded. Candidates out of range (or j == i) add exact
+    // zeros: their pressure weight is selected away, since a NaN pressure times
+    // a zero weight would not be zero. A kept candidate contributes as in the
+    // reference, a zero kernel weight times its terms included.
     auto pair_at = [&](uint32_t j, const float4 xj, const float4 vj) {
         const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
         const float r2 = dx * dx + dy * dy + dz * dz;
-        const bool ok = j != i && r2 <= h2;
+        const bool ok = j != i && within_h2(r2, h2);
         const float r = __builtin_amdgcn_sqrtf(r2);
         const bool zero = r == 0.f;  // r2 == 0 implies r == 0
         const float inv = zero ? 0.f : __builtin_amdgcn_rcpf(r);
         const float q = h - r;
-        const bool in = ok && r < h;
+        const bool in = inside_h(r, h);
         const float w1 = in ? -3.0f * q * q : 0.f;
-        const float w2 = in ? 6.0f * q : 0.f;
-        const float sp = (pi + xj.w) * w1;
+        const float w2 = ok && in ? 6.0f * q : 0.f;
+        const float sp = ok ? (pi + xj.w) * w1 : 0.f;
         fpx.add((dx * inv) * sp);
         fpy.add((dy * inv) * sp);
         fpz.add((dz * inv) * sp);
@@ -693,9 +730,10 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const Block
 #pragma unroll
         for (uint32_t u = 0; u < kBatchForce; ++u) pair_at(j[u], xj[u], vj[u]);
     };
-    uint32_t noWalls = 0u;  // walls come from the density pass's sums
+    uint32_t noWalls = 0u;  // walls come from the density pass's sums; kPoisonBit
+    const uint64_t nanLanes = ballot64(valid && (xi.x != xi.x || xi.y != xi.y || xi.z != xi.z));
     for_each_group<false>(
-        lane, wave, valid, code, cx, cy, cz, S, table, nullptr, lo, hi, gt, noWalls,
+        lane, wave, valid, code, cx, cy, cz, S, table, nullptr, nanCnt[bd.scene], nanLanes, lo, hi, gt, noWalls,
         [&](uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, auto&& between) { between(); },
         [&](uint32_t slot, uint32_t crowded) {
             walk_mask<kBatchForce>(m, gt.start[wave][slot], i, pair4);
@@ -760,7 +798,11 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const Block
             Fz += f.z;
         }
     }
-    const float ax = Fx / rho + S.gx, ay = Fy / rho + S.gy, az = Fz / rho + S.gz;
+    // A poisoned particle's acceleration is NaN, as the pile it skipped makes it.
+    const float nan = __builtin_nanf("");
+    const bool poisoned = (noWalls & kPoisonBit) != 0u;
+    const float ax = poisoned ? nan : Fx / rho + S.gx, ay = poisoned ? nan : Fy / rho + S.gy,
+                az = poisoned ? nan : Fz / rho + S.gz;
     if (kDebug) accOut[i] = make_float4(ax, ay, az, 0.f);
     // SF:641-645: symplectic Euler, then the key of the new cell.
     const float nvx = vi.x + ax * dt, nvy = vi.y + ay * dt, nvz = vi.z + az * dt;
@@ -772,6 +814,10 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const Block
     const uint32_t gk = S.tBegin + key;
     const Run run = wave_run(gk);
     if (run.is_head) atomicAdd(&counts[gk], run.len);
+    // Next step's count of particles with a NaN coordinate in bucket 0. Lane 0
+    // is active: the active lanes are a prefix of the wave.
+    const uint64_t nb = ballot64(key == 0u && (nx != nx || ny != ny || nz != nz));
+    if (nb != 0ull && lane == 0u) atomicAdd(&nanNext[bd.scene], (uint32_t)__popcll(nb));
 }
 
 }  // namespace sph

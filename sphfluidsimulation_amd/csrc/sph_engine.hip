@@ -74,6 +74,7 @@ struct sph_engine {
     float4* acc = nullptr;
     uint32_t* mask = nullptr;       // candidate mask from the density pass (run, tile and cell-wave walks)
     uint32_t* extraMask = nullptr;  // cell-wave: neighbours among crowded buckets' extra particles
+    uint32_t* nanCnt = nullptr;     // 2 x nScenes: NaN particles in bucket 0, per parity
     double* consPartial = nullptr;
     double* consOut = nullptr;
     uint32_t* valBits = nullptr;
@@ -172,6 +173,7 @@ void release(sph_engine* h) {
     dfree(h->acc);
     dfree(h->mask);
     dfree(h->extraMask);
+    dfree(h->nanCnt);
     dfree(h->consPartial);
     dfree(h->consOut);
     dfree(h->valBits);
@@ -294,16 +296,19 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     if ((rc = begin(SPH_STAGE_DENSITY))) return rc;
     const Walk walk = walk_of(h->flags);
     const bool mixed = (h->flags & SPH_FLAG_FP16_OFFSETS) != 0;  // cell-wave walk only
+    // Per-scene counts of NaN particles in bucket 0: read this step, accumulated for the next.
+    const uint32_t* nanNow = h->nanCnt + (size_t)c * h->nScenes;
+    uint32_t* nanNext = h->nanCnt + (size_t)(c ^ 1) * h->nScenes;
     if (walk != Walk::Cells) {
 #define SPH_DENS(S, D)                                                                                         \
     if (walk == Walk::CellWave && mixed)                                                                       \
         hipLaunchKernelGGL((k_density_cw<S, D, true>), gridX, block, 0, h->stream, h->dBlocks,            \
                            h->numBlocks, h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->wallMap,        \
-                           h->tileOrder, h->heavy, h->wallSum, h->wallZ, h->mask, h->extraMask, h->dens);     \
+                           h->tileOrder, h->heavy, h->wallSum, h->wallZ, h->mask, h->extraMask, nanNow, h->dens); \
     else if (walk == Walk::CellWave)                                                                           \
         hipLaunchKernelGGL((k_density_cw<S, D, false>), gridX, block, 0, h->stream, h->dBlocks,            \
                            h->numBlocks, h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->wallMap,        \
-                           h->tileOrder, h->heavy, h->wallSum, h->wallZ, h->mask, h->extraMask, h->dens);     \
+                           h->tileOrder, h->heavy, h->wallSum, h->wallZ, h->mask, h->extraMask, nanNow, h->dens); \
     else if (walk == Walk::Tiles)                                                                              \
         hipLaunchKernelGGL((k_density_tile<S, D>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,       \
                            h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->mask, h->dens);                 \
@@ -349,13 +354,13 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     if (walk == Walk::CellWave && mixed)                                                                          \
         hipLaunchKernelGGL((k_force_cw<S, D, T, true>), gridX, block, 0, h->stream, h->dBlocks,                \
                            h->numBlocks, h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->tileOrder,  \
-                           h->wallSum, h->wallZ, h->mask, h->extraMask, h->ids[c], h->tris, dt, h->posT,         \
-                           h->velT, h->tab[c ^ 1], h->acc);                                                       \
+                           h->wallSum, h->wallZ, h->mask, h->extraMask, nanNow, nanNext, h->ids[c], h->tris, dt, \
+                           h->posT, h->velT, h->tab[c ^ 1], h->acc);                                              \
     else if (walk == Walk::CellWave)                                                                              \
         hipLaunchKernelGGL((k_force_cw<S, D, T, false>), gridX, block, 0, h->stream, h->dBlocks,                \
                            h->numBlocks, h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->tileOrder,  \
-                           h->wallSum, h->wallZ, h->mask, h->extraMask, h->ids[c], h->tris, dt, h->posT,         \
-                           h->velT, h->tab[c ^ 1], h->acc);                                                       \
+                           h->wallSum, h->wallZ, h->mask, h->extraMask, nanNow, nanNext, h->ids[c], h->tris, dt, \
+                           h->posT, h->velT, h->tab[c ^ 1], h->acc);                                              \
     else if (walk == Walk::Tiles)                                                                                 \
         hipLaunchKernelGGL((k_force_tile<S, D, T>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,         \
                            h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->mask, h->ids[c], h->tris,  \
@@ -389,7 +394,8 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     // Scan the histogram; clear the table read this step for the next histogram.
     if ((rc = begin(SPH_STAGE_SCAN))) return rc;
     hipLaunchKernelGGL(k_scan, dim3(h->numTiles), dim3(kScanThreads), 0, h->stream, h->tab[c ^ 1], h->tab[c],
-                       (uint32_t)h->L, h->status[c], h->status[c ^ 1], h->ticket, h->numTiles);
+                       (uint32_t)h->L, h->status[c], h->status[c ^ 1], h->ticket, h->numTiles,
+                       h->nanCnt + (size_t)c * h->nScenes, h->nScenes);
     SPH_HIP(h, hipGetLastError());
     if ((rc = end())) return rc;
 
@@ -524,7 +530,7 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
     h->L = L;
 
     std::vector<float4> hPos(N), hVel(N), hSpos(Ns);
-    std::vector<uint32_t> hIds(N), hOrig(N), hTab(L), hStab, hStaticScene(Ns), hWallMap;
+    std::vector<uint32_t> hIds(N), hOrig(N), hTab(L), hStab, hStaticScene(Ns), hWallMap, hNan(n_scenes, 0u);
     h->hStaticIds.assign(Ns, 0);
     h->hScenes.resize(n_scenes);
     h->hSceneBlocks.assign(n_scenes + 1, 0);
@@ -603,6 +609,10 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
                 hOrig[g] = (uint32_t)(p0 + i);
             }
             for (uint64_t b = 0; b <= S.M; ++b) hTab[tBegin + b] = (uint32_t)(p0 + table[b]);
+            for (uint64_t i = 0; i < pn; ++i) {
+                const float* q = dyn_pos + 3 * (p0 + i);
+                if (keys[i] == 0u && (std::isnan(q[0]) || std::isnan(q[1]) || std::isnan(q[2]))) ++hNan[s];
+            }
         } else {
             hTab[tBegin] = (uint32_t)p0;
         }
@@ -654,6 +664,7 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
         SPH_ALLOC(h->wallZ, std::max<uint64_t>(N, 1));
     }
     SPH_ALLOC(h->staticScene, std::max<uint64_t>(Ns, 1));
+    SPH_ALLOC(h->nanCnt, 2ull * n_scenes);
     SPH_ALLOC(h->consPartial, 8ull * std::max<uint32_t>(h->numBlocks, 1));
     SPH_ALLOC(h->consOut, 8ull * n_scenes);
     SPH_ALLOC(h->valBits, 1);
@@ -682,6 +693,8 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
         SPH_HIP(h, hipMemset(h->pres, 0, sizeof(float) * N));
     }
     SPH_HIP(h, hipMemcpy(h->tab[0], hTab.data(), sizeof(uint32_t) * L, hipMemcpyHostToDevice));
+    hNan.resize(2ull * n_scenes, 0u);  // parity 1 starts at zero
+    SPH_HIP(h, hipMemcpy(h->nanCnt, hNan.data(), sizeof(uint32_t) * hNan.size(), hipMemcpyHostToDevice));
     SPH_HIP(h, hipMemset(h->tab[1], 0, sizeof(uint32_t) * L));
     SPH_HIP(h, hipMemset(h->status[0], 0, sizeof(unsigned long long) * h->numTiles));
     SPH_HIP(h, hipMemset(h->status[1], 0, sizeof(unsigned long long) * h->numTiles));
@@ -1121,7 +1134,7 @@ int sph_autotune(sph_handle h, float dt, uint32_t trial_steps, uint32_t* chosen)
     const uint32_t savedProfiling = h->profiling;
     h->profiling = 0;
     float4 *bPos = nullptr, *bVel = nullptr;
-    uint32_t *bIds = nullptr, *bTab = nullptr;
+    uint32_t *bIds = nullptr, *bTab = nullptr, *bNan = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = SPH_OK;
     auto finish = [&](int code) {
@@ -1129,16 +1142,19 @@ int sph_autotune(sph_handle h, float dt, uint32_t trial_steps, uint32_t* chosen)
         dfree(bVel);
         dfree(bIds);
         dfree(bTab);
+        dfree(bNan);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
         h->profiling = savedProfiling;
         return code;
     };
     // Snapshot of the state a step starts from: tab[c] holds the cell starts,
-    // tab[c ^ 1] is all zero, and the scan's status arrays are in the state
-    // parity c expects after any even number of steps.
+    // tab[c ^ 1] is all zero, the NaN counts of parity c are this state's and
+    // those of parity c ^ 1 are zero, and the scan's status arrays are in the
+    // state parity c expects after any even number of steps.
+    const uint64_t S2 = 2ull * h->nScenes;
     if ((rc = dalloc(h, &bPos, N)) || (rc = dalloc(h, &bVel, N)) || (rc = dalloc(h, &bIds, N)) ||
-        (rc = dalloc(h, &bTab, L)))
+        (rc = dalloc(h, &bTab, L)) || (rc = dalloc(h, &bNan, S2)))
         return finish(rc);
     auto copy = [&](void* dst, const void* src, uint64_t bytes) {
         return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, h->stream);
@@ -1149,12 +1165,14 @@ int sph_autotune(sph_handle h, float dt, uint32_t trial_steps, uint32_t* chosen)
         if ((e = copy(h->vel, bVel, N * sizeof(float4))) != hipSuccess) return e;
         if ((e = copy(h->ids[c], bIds, N * sizeof(uint32_t))) != hipSuccess) return e;
         if ((e = copy(h->tab[c], bTab, L * sizeof(uint32_t))) != hipSuccess) return e;
+        if ((e = copy(h->nanCnt, bNan, S2 * sizeof(uint32_t))) != hipSuccess) return e;
         return hipMemsetAsync(h->tab[c ^ 1], 0, L * sizeof(uint32_t), h->stream);
     };
     hipError_t e = copy(bPos, h->pos, N * sizeof(float4));
     if (e == hipSuccess) e = copy(bVel, h->vel, N * sizeof(float4));
     if (e == hipSuccess) e = copy(bIds, h->ids[c], N * sizeof(uint32_t));
     if (e == hipSuccess) e = copy(bTab, h->tab[c], L * sizeof(uint32_t));
+    if (e == hipSuccess) e = copy(bNan, h->nanCnt, S2 * sizeof(uint32_t));
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     if (e != hipSuccess) {

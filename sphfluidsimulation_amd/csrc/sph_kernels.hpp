@@ -191,7 +191,7 @@ __global__ __launch_bounds__(kBlock) void k_density(const BlockDesc* __restrict_
                     if (r2 > h2) continue;
                     const float r = __builtin_amdgcn_sqrtf(r2);
                     const float q = h - r;
-                    sumDyn += (r < h) ? q * q * q : 0.f;
+                    sumDyn += inside_h(r, h) ? q * q * q : 0.f;
                 }
                 if (kStatic && S.sCount) {
                     const uint32_t sb = S.stBegin + fastmod(code, S.magicMs, S.Ms);
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(kBlock) void k_density(const BlockDesc* __restrict_
                         if (r2 > h2) continue;
                         const float r = __builtin_amdgcn_sqrtf(r2);
                         const float q = h - r;
-                        sumSt += (r < h) ? q * q * q : 0.f;
+                        sumSt += inside_h(r, h) ? q * q * q : 0.f;
                     }
                 }
             }
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(kBlock) void k_force(const BlockDesc* __restrict__ 
                         dz *= inv;
                     }
                     const float q = h - r;
-                    const bool in = r < h;
+                    const bool in = inside_h(r, h);
                     const float w1 = in ? -3.0f * q * q : 0.f;
                     const float w2 = in ? 6.0f * q : 0.f;
                     const float pj = pres[j];
@@ -302,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void k_force(const BlockDesc* __restrict__ 
                             dz *= inv;
                         }
                         const float q = h - r;
-                        const bool in = r < h;
+                        const bool in = inside_h(r, h);
                         const float sp = absP * (in ? -3.0f * q * q : 0.f);
                         fsx += dx * sp;
                         fsy += dy * sp;
@@ -396,16 +396,16 @@ __global__ __launch_bounds__(kBlock) void k_density_runs(const BlockDesc* __rest
                           const float dx = xs[q].x - xi.x, dy = xs[q].y - xi.y, dz = xs[q].z - xi.z;
                           const float r2 = dx * dx + dy * dy + dz * dz;
                           const bool self = j + q == i;
-                          if (r2 <= h2wide && !self) bits |= 1u << (k & 31u);
+                          if (within_h2(r2, h2wide) && !self) bits |= 1u << (k & 31u);
                           if ((k & 31u) == 31u) {
                               if (k < kMaskBits) smask[(k >> 5) * kBlock + tid] = bits;
                               bits = 0u;
                           }
                           ++k;
-                          if (!self && r2 <= h2) {
+                          if (!self && within_h2(r2, h2)) {
                               const float r = __builtin_amdgcn_sqrtf(r2);
                               const float qh = h - r;
-                              sumDyn += (r < h) ? qh * qh * qh : 0.f;
+                              sumDyn += inside_h(r, h) ? qh * qh * qh : 0.f;
                           }
                       }
                   }
@@ -421,7 +421,7 @@ __global__ __launch_bounds__(kBlock) void k_density_runs(const BlockDesc* __rest
                           if (r2 > h2) continue;
                           const float r = __builtin_amdgcn_sqrtf(r2);
                           const float q = h - r;
-                          sumSt += (r < h) ? q * q * q : 0.f;
+                          sumSt += inside_h(r, h) ? q * q * q : 0.f;
                       }
                   });
     }
@@ -497,7 +497,7 @@ __global__ __launch_bounds__(kBlock) void k_force_runs(const BlockDesc* __restri
             dz *= inv;
         }
         const float q = h - r;
-        const bool in = r < h;
+        const bool in = inside_h(r, h);
         const float w1 = in ? -3.0f * q * q : 0.f;
         const float w2 = in ? 6.0f * q : 0.f;
         const float sp = (pi + xj.w) * w1;
@@ -566,7 +566,7 @@ __global__ __launch_bounds__(kBlock) void k_force_runs(const BlockDesc* __restri
                               dz *= inv;
                           }
                           const float q = h - r;
-                          const bool in = r < h;
+                          const bool in = inside_h(r, h);
                           const float sp = absP * (in ? -3.0f * q * q : 0.f);
                           fsx += dx * sp;
                           fsy += dy * sp;
@@ -611,8 +611,9 @@ __global__ __launch_bounds__(kBlock) void k_force_runs(const BlockDesc* __restri
 // tile only waits on tiles that are already running. Status words carry a
 // 2-bit flag (1 = aggregate, 2 = inclusive prefix) over a 32-bit value in one
 // 64-bit atomic. The kernel also zeroes `zero_out` (the other table, which
-// collects the next step's histogram) and the other status array, so no
-// memset is needed between steps and the launch can live in a hipGraph.
+// collects the next step's histogram), the other status array and the n_clear
+// words of `clear` (the per-scene counts the next force pass accumulates), so
+// no memset is needed between steps and the launch can live in a hipGraph.
 constexpr int kScanThreads = 256;
 constexpr int kScanRows = 4;
 constexpr uint32_t kScanTile = kScanThreads * 4u * kScanRows;  // 4096 entries per tile
@@ -644,7 +645,8 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t& total, uin
 __global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* __restrict__ data, uint32_t* __restrict__ zero_out,
                                                        uint32_t n, unsigned long long* __restrict__ status,
                                                        unsigned long long* __restrict__ status_clear,
-                                                       uint32_t* __restrict__ ticket, uint32_t num_tiles) {
+                                                       uint32_t* __restrict__ ticket, uint32_t num_tiles,
+                                                       uint32_t* __restrict__ clear, uint32_t n_clear) {
     __shared__ uint32_t s_tile, s_prefix;
     __shared__ uint32_t s_w[4];
     const uint32_t tid = threadIdx.x;
@@ -729,6 +731,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* __restrict__ da
             }
         }
     }
+    if (tile == 0u)
+        for (uint32_t t = tid; t < n_clear; t += kScanThreads) clear[t] = 0u;
 }
 
 // ---------------------------------------------------------------------------

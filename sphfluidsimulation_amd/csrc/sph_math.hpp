@@ -64,6 +64,14 @@ __host__ __device__ inline float kernel_constant(float h) {
     return 15.0f / (3.1415f * h * h * h * h * h * h);
 }
 
+// The reference drops a candidate only when distSqr > h^2 (SF:447, SF:488,
+// SF:571, SF:605), so a NaN distance is kept, and its kernels return 0 only
+// when r >= h (SF:119-139), so a NaN r gives NaN weights: a particle with a NaN
+// coordinate poisons every particle that visits its bucket. These two tests
+// keep that; for numbers they are r2 <= h2 and r < h.
+__host__ __device__ inline bool within_h2(float r2, float h2) { return !(r2 > h2); }
+__host__ __device__ inline bool inside_h(float r, float h) { return !(r >= h); }
+
 // SF:119-125 at r = 0, for selfDensity = m * D0(0, h) * C (SimulationEngineCPU.cpp:407).
 __host__ __device__ inline float d0_at_zero(float h) { return (h - 0.0f) * (h - 0.0f) * (h - 0.0f); }
 

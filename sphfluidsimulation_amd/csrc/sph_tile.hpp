@@ -217,7 +217,7 @@ __global__ __launch_bounds__(kBlock) void k_density_tile(const BlockDesc* __rest
             for (uint32_t q = 0; q < 4; ++q) {
                 const float dx = xs[q].x - xi.x, dy = xs[q].y - xi.y, dz = xs[q].z - xi.z;
                 const float r2 = dx * dx + dy * dy + dz * dz;
-                const bool in = (t + q < n8) && (t + q != selfT) && r2 <= h2wide;
+                const bool in = (t + q < n8) && (t + q != selfT) && within_h2(r2, h2wide);
                 bits8 |= in ? (1u << (t + q)) : 0u;
             }
         }
@@ -225,10 +225,10 @@ __global__ __launch_bounds__(kBlock) void k_density_tile(const BlockDesc* __rest
             const float4 xj = src[t];
             const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 <= h2 && t != selfT) {
+            if (within_h2(r2, h2) && t != selfT) {
                 const float r = __builtin_amdgcn_sqrtf(r2);
                 const float q = h - r;
-                sumDyn += (r < h) ? q * q * q : 0.f;
+                sumDyn += inside_h(r, h) ? q * q * q : 0.f;
             }
         };
         for (uint32_t mb = bits8; mb; mb &= mb - 1u) d0((uint32_t)__builtin_ctz(mb));
@@ -260,7 +260,7 @@ __global__ __launch_bounds__(kBlock) void k_density_tile(const BlockDesc* __rest
                           if (r2 > h2) continue;
                           const float r = __builtin_amdgcn_sqrtf(r2);
                           const float q = h - r;
-                          sumSt += (r < h) ? q * q * q : 0.f;
+                          sumSt += inside_h(r, h) ? q * q * q : 0.f;
                       }
                   });
     }
@@ -347,7 +347,7 @@ __global__ __launch_bounds__(kBlock) void k_force_tile(const BlockDesc* __restri
             dz *= inv;
         }
         const float q = h - r;
-        const bool in = r < h;
+        const bool in = inside_h(r, h);
         const float w1 = in ? -3.0f * q * q : 0.f;
         const float w2 = in ? 6.0f * q : 0.f;
         const float sp = (pi + xj.w) * w1;
@@ -400,7 +400,7 @@ __global__ __launch_bounds__(kBlock) void k_force_tile(const BlockDesc* __restri
                               dz *= inv;
                           }
                           const float q = h - r;
-                          const bool in = r < h;
+                          const bool in = inside_h(r, h);
                           const float sp = absP * (in ? -3.0f * q * q : 0.f);
                           fsx += dx * sp;
                           fsy += dy * sp;

@@ -20,7 +20,8 @@ import oracle as orc
 from helpers import bucket_multisets_equal, check_buckets, expected_starts, max_norm_rel, max_rel
 from sphfluidsimulation_amd import SPHError, SimulationEngineHIP, scenes
 from sphfluidsimulation_amd._lib import (SPH_E_STATE, SPH_FLAG_CELL_WALK, SPH_FLAG_FP16_OFFSETS, SPH_FLAG_HIP_GRAPH,
-                                         SPH_FLAG_RECORD_DEBUG, SPH_FLAG_RUN_WALK, SPH_FLAG_TILE_WALK)
+                                         SPH_FLAG_RECORD_DEBUG, SPH_FLAG_RUN_WALK, SPH_FLAG_TILE_WALK,
+                                         SPH_V_NONFINITE)
 
 pytestmark = pytest.mark.gpu
 
@@ -50,7 +51,7 @@ def s1m():
     return scenes.gpu_1m()
 
 
-def _init_parity(sc, threads=1):
+def _init_parity(sc, threads=1, nonfinite=False):
     e = SimulationEngineHIP(0)
     e.Initialize(sc)
     ref = orc.Oracle(sc.params, sc.pos, sc.vel, sc.static_pos, threads=threads)
@@ -67,7 +68,7 @@ def _init_parity(sc, threads=1):
         rst, rsid = ref.read_static()
         np.testing.assert_array_equal(st[0], rst)
         np.testing.assert_array_equal(sid, rsid)
-    assert e.validate() == 0
+    assert e.validate() & ~(SPH_V_NONFINITE if nonfinite else 0) == 0
     pos, vel, p = e.read()
     np.testing.assert_array_equal(pos, sc.pos)
     np.testing.assert_array_equal(vel, sc.vel)
@@ -109,7 +110,7 @@ def _check_step(sc, got, ref, dt):
                                                  + 4 * np.finfo(np.float32).eps * (1 + float(np.max(np.abs(rpos)))))
 
 
-def _step_integer_parity(e: SimulationEngineHIP, sc):
+def _step_integer_parity(e: SimulationEngineHIP, sc, nonfinite=False):
     """Keys, table and buckets of the GPU's own state after its steps, against
     the oracle applied to the same positions."""
     pos, _, _ = e.read()
@@ -124,7 +125,7 @@ def _step_integer_parity(e: SimulationEngineHIP, sc):
     ref = orc.Oracle(sc.params, pos, None, sc.static_pos, threads=1)
     assert bucket_multisets_equal(starts, ids, ref.read_hash_map(), ref.read_bucket_ids())
     ref.close()
-    assert e.validate() == 0
+    assert e.validate() & ~(SPH_V_NONFINITE if nonfinite else 0) == 0
 
 
 def test_init_parity_16k(s16k):
@@ -321,6 +322,7 @@ def _occupancy_scene(kind):
     # particles of cells 1024 apart;
     # crowded: 27 particles per cell on average, so every bucket holds more than
     # the 8 candidates per bucket the cell-group layout keeps in registers;
+    # dense: about 11 per cell;
     # sparse: about 2 per cell, so a wave of 64 particles spans more than the 32
     # cell groups whose bucket tables it keeps at once.
     if kind == "aliased":
@@ -335,11 +337,17 @@ def _occupancy_scene(kind):
     if kind == "crowded":
         return scenes.dam_break(24, 24, 24, (2.4, 2.4, 2.4), spacing=0.1 / 3.0, jitter_frac=0.3, seed=11,
                                 shuffle=True, name="crowded-cells")
+    if kind == "dense":
+        # About 11 per cell: a particle's extra candidates past the 8 per bucket
+        # number from a few to past 32, so both of the force pass's ways of
+        # taking them (the density pass's record, or every one) run.
+        return scenes.dam_break(26, 26, 26, (2.4, 2.4, 2.4), spacing=0.1 / 2.2, jitter_frac=0.3, seed=14,
+                                shuffle=True, name="dense-cells")
     return scenes.dam_break(30, 30, 30, (4.0, 4.0, 4.0), spacing=0.08, jitter_frac=0.2, seed=12, shuffle=True,
                             name="sparse-cells")
 
 
-@pytest.mark.parametrize("kind", ["crowded", "sparse", "aliased"])
+@pytest.mark.parametrize("kind", ["crowded", "dense", "sparse", "aliased"])
 @pytest.mark.parametrize("walk", WALKS)
 def test_cell_occupancy_extremes(kind, walk):
     say(f"{kind} cells (walk flags {walk})")
@@ -347,6 +355,76 @@ def test_cell_occupancy_extremes(kind, walk):
     _init_parity(sc)
     got, ref = _one_step(sc, 1e-3, threads=4, walk=walk)
     _check_step(sc, got, ref, 1e-3)
+
+
+def _nan_scene(kind):
+    # pile: 30 particles with a NaN position (all in cell 0, bucket 0, at the
+    # fluid's corner) and 4 with a NaN x only; single: one NaN position, so the
+    # group holding it walks bucket 0; partial: 3 particles with one NaN
+    # coordinate each, in buckets other than 0.
+    base = scenes.cpu_16k()
+    pos = base.pos.copy()
+    idx = np.random.default_rng(5).choice(base.n, 40, replace=False)
+    if kind == "pile":
+        pos[idx[:30]] = np.nan
+        pos[idx[30:34], 0] = np.nan
+    elif kind == "single":
+        pos[idx[0]] = np.nan
+    else:
+        for a in range(3):
+            pos[idx[a], a] = np.nan
+    return scenes.Scene(base.params, pos, base.vel, base.static_pos, name=f"nan-{kind}")
+
+
+def _check_step_nan(sc, got, ref, dt):
+    """The reference keeps NaN distances (distSqr > h^2 is false), so a NaN
+    position poisons every particle that visits its bucket: the NaN pattern of
+    every output must match exactly, and the finite rows meet the usual bars."""
+    bad = np.zeros(sc.n, bool)
+    for g, r in zip(got, ref):
+        gn, rn = np.isnan(g), np.isnan(r)
+        np.testing.assert_array_equal(gn, rn)
+        bad |= rn.reshape(sc.n, -1).any(axis=1)
+    say(f"{sc.name}: {int(bad.sum())} particles NaN after the step")
+    ok = ~bad
+    d, a, pos, vel, p = (x[ok] for x in got)
+    rd, ra, rpos, rvel, rp = (x[ok] for x in ref)
+    k = sc.params.gasConstant
+    assert max_rel(d, rd) <= DENS_TOL
+    assert float(np.max(np.abs(p - rp) / (k * rd))) <= DENS_TOL
+    assert max_norm_rel(a, ra) <= ACC_TOL
+    amax = float(np.max(np.abs(ra)))
+    assert float(np.max(np.abs(vel - rvel))) <= ACC_TOL * amax * dt + 4 * np.finfo(np.float32).eps * (
+        1 + float(np.max(np.abs(rvel))))
+    assert float(np.max(np.abs(pos - rpos))) <= (ACC_TOL * amax * dt * dt
+                                                 + 4 * np.finfo(np.float32).eps * (1 + float(np.max(np.abs(rpos)))))
+    return int(bad.sum())
+
+
+@pytest.mark.parametrize("kind", ["pile", "single", "partial"])
+@pytest.mark.parametrize("walk", WALKS)
+def test_nan_positions(kind, walk):
+    """NaN positions, two steps: the second walks the first step's NaN particles
+    (counted by the force pass), the first those the scene starts with."""
+    say(f"NaN positions: {kind} (walk flags {walk})")
+    sc = _nan_scene(kind)
+    _init_parity(sc, nonfinite=True)
+    dt = 1e-3
+    e = SimulationEngineHIP(0)
+    e.Initialize(sc, flags=SPH_FLAG_RECORD_DEBUG | walk)
+    ref = orc.Oracle(sc.params, sc.pos, sc.vel, sc.static_pos, threads=4, flags=orc.FLAG_RECORD_DEBUG)
+    counts = []
+    for _ in range(2):
+        e.Update(dt, 1)
+        ref.update(dt, 1)
+        got = (*e.read_debug(), *e.read())
+        exp = (*ref.read_debug(), *ref.read())
+        counts.append(_check_step_nan(sc, got, exp, dt))
+    _step_integer_parity(e, sc, nonfinite=True)
+    e.close()
+    ref.close()
+    # The poison spreads: more NaN particles after the second step than the scene's.
+    assert counts[1] >= counts[0] > 0
 
 
 def test_autotune_leaves_state_unchanged(s16k):
