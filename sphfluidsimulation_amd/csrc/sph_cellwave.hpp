@@ -112,7 +112,7 @@ constexpr uint32_t kBatchCrowded = SPH_CW_BATCH_CROWDED;
 constexpr uint32_t kBatchForce = SPH_CW_BATCH_FORCE;
 // Crowded buckets: the density pass records which of a particle's first
 // kExtraBits extra candidates are neighbours, for the force pass.
-constexpr uint32_t kExtraBits = 32;
+constexpr uint32_t kExtraBits = 64;
 
 // Walks the set bits of a neighbour mask B at a time: f(j) gets B candidate
 // indices (bucket start of the bit's bucket + slot), `self` standing in for
@@ -439,7 +439,7 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
                                                        uint32_t* __restrict__ heavy,
                                                        float4* __restrict__ wallSum, float4* __restrict__ wallZ,
                                                        uint32_t* __restrict__ mask,
-                                                       uint32_t* __restrict__ extraMask,
+                                                       uint64_t* __restrict__ extraMask,
                                                        const uint32_t* __restrict__ nanCnt, float* __restrict__ dens) {
     __shared__ GroupTables gt;
     const uint32_t tile = xcd_tile(blockIdx.x, numBlocks);
@@ -603,7 +603,8 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
             // visit order, is a neighbour; all ones when there are more than
             // kExtraBits extras. The force pass reads it.
             if (!crowded) return;
-            uint32_t k = 0, bits = 0u;
+            uint32_t k = 0;
+            uint64_t bits = 0ull;
             for (; crowded; crowded &= crowded - 1u) {
                 const uint32_t bi = (uint32_t)__builtin_ctz(crowded);
                 const uint32_t s0 = gt.start[wave][slot][bi], e = s0 + gt.cnt[wave][slot][bi];
@@ -612,11 +613,11 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
 #pragma unroll
                     for (uint32_t u = 0; u < kBatchCrowded; ++u) j[u] = jj + u < e ? jj + u : i;
                     const uint32_t hits = d4(j);  // padded entries (j = i) never hit
-                    if (k < kExtraBits) bits |= hits << k;
+                    if (k < kExtraBits) bits |= (uint64_t)hits << k;
                     k += min(kBatchCrowded, e - jj);
                 }
             }
-            extraMask[i] = k > kExtraBits ? ~0u : bits;
+            extraMask[i] = k > kExtraBits ? ~0ull : bits;
         });
     if (kStatic && ballot64(valid && (walls & ~kPoisonBit) != 0u) != 0ull && lane == 0u) heavy[blk] = 1u;
     if (!valid) return;
@@ -651,7 +652,7 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const Block
                                                      const float4* __restrict__ wallSum,
                                                      const float4* __restrict__ wallZ,
                                                      const uint32_t* __restrict__ mask,
-                                                     const uint32_t* __restrict__ extraMask,
+                                                     const uint64_t* __restrict__ extraMask,
                                                      const uint32_t* __restrict__ nanCnt, uint32_t* __restrict__ nanNext,
                                                      const uint32_t* __restrict__ ids, const float* __restrict__ tris,
                                                      float dt, float4* __restrict__ posT, float4* __restrict__ velT,
@@ -720,15 +721,16 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const Block
         fvy.add((vj.y - vi.y) * w2);
         fvz.add((vj.z - vi.z) * w2);
     };
-    auto pair4 = [&](const uint32_t (&j)[kBatchForce]) {
-        float4 xj[kBatchForce], vj[kBatchForce];
+    auto pair4 = [&](const auto& j) {
+        constexpr uint32_t B = sizeof(j) / sizeof(j[0]);
+        float4 xj[B], vj[B];
 #pragma unroll
-        for (uint32_t u = 0; u < kBatchForce; ++u) {
+        for (uint32_t u = 0; u < B; ++u) {
             xj[u] = ld16(pos, j[u]);
             vj[u] = ld16(vel, j[u]);
         }
 #pragma unroll
-        for (uint32_t u = 0; u < kBatchForce; ++u) pair_at(j[u], xj[u], vj[u]);
+        for (uint32_t u = 0; u < B; ++u) pair_at(j[u], xj[u], vj[u]);
     };
     uint32_t noWalls = 0u;  // walls come from the density pass's sums; kPoisonBit
     const uint64_t nanLanes = ballot64(valid && (xi.x != xi.x || xi.y != xi.y || xi.z != xi.z));
@@ -741,23 +743,18 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const Block
             // among them when there are at most kExtraBits, else every one, a
             // batch at a time.
             if (!crowded) return;
-            uint32_t xb = extraMask[i];
-            const bool all = xb == ~0u;
+            uint64_t xb = extraMask[i];
+            const bool all = xb == ~0ull;
             for (; crowded; crowded &= crowded - 1u) {
                 const uint32_t bi = (uint32_t)__builtin_ctz(crowded);
                 const uint32_t s0 = gt.start[wave][slot][bi] + kSlots, ne = gt.cnt[wave][slot][bi] - kSlots;
                 for (uint32_t base = 0; base < ne; base += 32u) {
                     const uint32_t c = min(32u, ne - base);
                     const uint32_t low = c < 32u ? (1u << c) - 1u : ~0u;
-                    uint32_t w = all ? low : xb & low;
-                    xb = c < 32u ? xb >> c : 0u;
-                    while (w) {
-                        uint32_t j[kBatchForce];
-#pragma unroll
-                        for (uint32_t u = 0; u < kBatchForce; ++u) {
-                            j[u] = w ? s0 + base + (uint32_t)__builtin_ctz(w) : i;
-                            w &= w - 1u;
-                        }
+                    uint32_t w = all ? low : (uint32_t)xb & low;
+                    xb >>= c;
+                    for (; w; w &= w - 1u) {  // one at a time: fewer registers
+                        const uint32_t j[1] = {s0 + base + (uint32_t)__builtin_ctz(w)};
                         pair4(j);
                     }
                 }

@@ -73,7 +73,7 @@ struct sph_engine {
     float* dens = nullptr;
     float4* acc = nullptr;
     uint32_t* mask = nullptr;       // candidate mask from the density pass (run, tile and cell-wave walks)
-    uint32_t* extraMask = nullptr;  // cell-wave: neighbours among crowded buckets' extra particles
+    uint64_t* extraMask = nullptr;  // cell-wave: neighbours among crowded buckets' extra particles
     uint32_t* nanCnt = nullptr;     // 2 x nScenes: NaN particles in bucket 0, per parity
     double* consPartial = nullptr;
     double* consOut = nullptr;
