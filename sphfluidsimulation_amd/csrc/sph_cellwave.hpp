@@ -101,7 +101,7 @@ __device__ inline void settle(uint32_t (&j)[8]) {
 #define SPH_CW_BATCH_DENSITY 4
 #endif
 #ifndef SPH_CW_BATCH_FORCE
-#define SPH_CW_BATCH_FORCE 2
+#define SPH_CW_BATCH_FORCE 4
 #endif
 constexpr uint32_t kBatchDensity = SPH_CW_BATCH_DENSITY;
 // Density pass, crowded buckets' extra particles.
@@ -110,9 +110,15 @@ constexpr uint32_t kBatchDensity = SPH_CW_BATCH_DENSITY;
 #endif
 constexpr uint32_t kBatchCrowded = SPH_CW_BATCH_CROWDED;
 constexpr uint32_t kBatchForce = SPH_CW_BATCH_FORCE;
-// Crowded buckets: the density pass records which of a particle's first
-// kExtraBits extra candidates are neighbours, for the force pass.
-constexpr uint32_t kExtraBits = 64;
+// Neighbour lists. The density pass records, for the force pass, every
+// candidate it keeps (within_h2, j != i) in visit order: up to kListCap per
+// particle, slot n of the particle on lane l of wave w (workgroup b, wave
+// index q: w = b * kWaves + q) at nbr[(w * kListCap + n) * 64 + l], so a wave
+// reads or writes one slot of its 64 particles as one 256-byte row. The count
+// goes to nbrCount; kListPoison marks a poisoned particle (NaN results), and a
+// count above kListCap makes the force pass walk the 27 buckets itself.
+constexpr uint32_t kListCap = 64;
+constexpr uint32_t kListPoison = 0xFFFFFFFFu;
 
 // Walks the set bits of a neighbour mask B at a time: f(j) gets B candidate
 // indices (bucket start of the bit's bucket + slot), `self` standing in for
@@ -428,19 +434,30 @@ __global__ __launch_bounds__(256) void k_tile_order(uint32_t* __restrict__ heavy
     }
 }
 
+// Where the density pass writes its per-particle results (read once, at the
+// end of the pass, so these pointers hold no registers during the walk).
+struct DensityOut {
+    uint32_t* heavy;     // workgroups with walls in reach
+    uint32_t* nbrCount;  // neighbour list lengths
+    float4* wallSum;     // walls' geometric sums
+    float4* wallZ;       // coincident walls' w1
+    float* dens;         // density (debug recording)
+};
+
 template <bool kStatic, bool kDebug, bool kMixed>
-__global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
+#ifndef SPH_CW_DENSITY_WAVES
+#define SPH_CW_DENSITY_WAVES 7
+#endif
+__global__ __launch_bounds__(kBlock, kMixed ? 6 : SPH_CW_DENSITY_WAVES) void k_density_cw(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
                                                        const SceneDev* __restrict__ scenes, float4* pos,
                                                        const uint32_t* __restrict__ tab,
                                                        const float4* __restrict__ spos,
                                                        const uint32_t* __restrict__ stab,
                                                        const uint32_t* __restrict__ wallMap,
                                                        const uint32_t* __restrict__ order,
-                                                       uint32_t* __restrict__ heavy,
-                                                       float4* __restrict__ wallSum, float4* __restrict__ wallZ,
-                                                       uint32_t* __restrict__ mask,
-                                                       uint64_t* __restrict__ extraMask,
-                                                       const uint32_t* __restrict__ nanCnt, float* __restrict__ dens) {
+                                                       uint32_t* __restrict__ nbr,
+                                                       const uint32_t* __restrict__ nanCnt,
+                                                       const DensityOut* __restrict__ out) {
     __shared__ GroupTables gt;
     const uint32_t tile = xcd_tile(blockIdx.x, numBlocks);
     if (tile >= numBlocks) return;
@@ -481,13 +498,30 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
         tuy = h2_bits(pack_h2(uy, uy));
         tuz = h2_bits(pack_h2(uz, uz));
     }
-    // Exact contributions of four candidates j (SF:426-453).
-    // Returns bit u set for each candidate within h (r2 <= h2, j != i).
-    auto d4 = [&](const auto& j) -> uint32_t {
+    // Exact contributions of a batch of candidates j (SF:426-453). The kept ones
+    // (r2 <= h2, j != i) are appended to this particle's list one batch later,
+    // after the next batch's loads are issued, so the wait for those loads does
+    // not include the stores.
+    uint32_t* const list = nbr + (size_t)(blk * kWaves + wave) * (kListCap * 64u) + lane;
+    uint32_t nc = 0;                      // kept candidates so far
+    uint32_t pend[4] = {0u, 0u, 0u, 0u};  // the last batch, not yet appended
+    uint32_t pendHits = 0u;               // its kept candidates (bit u)
+    auto flush = [&]() {
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            const bool hit = (pendHits >> u) & 1u;
+            if (hit && nc < kListCap) list[nc << 6] = pend[u];
+            nc += hit ? 1u : 0u;
+        }
+        pendHits = 0u;
+    };
+    auto dn = [&](const auto& j) {
         constexpr uint32_t B = sizeof(j) / sizeof(j[0]);
+        static_assert(B <= 4, "at most four pending candidates");
         float4 xj[B];
 #pragma unroll
         for (uint32_t u = 0; u < B; ++u) xj[u] = ld16(pos, j[u]);
+        flush();
         uint32_t hits = 0u;
 #pragma unroll
         for (uint32_t u = 0; u < B; ++u) {
@@ -498,8 +532,9 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
             const bool ok = within_h2(r2, h2) && j[u] != i;
             sumDyn.add((ok && inside_h(r, h)) ? q * q * q : 0.f);
             hits |= ok ? 1u << u : 0u;
+            pend[u] = j[u];
         }
-        return hits;
+        pendHits = hits;
     };
 
     // Static neighbours (SF:455-496).
@@ -533,7 +568,7 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
     for_each_group<kStatic>(
         lane, wave, valid, code, cx, cy, cz, S, table, wallMap, nanCnt[bd.scene], nanLanes, lo, hi, gt, walls,
         [&](uint32_t, uint32_t g0, uint32_t g1, uint32_t start27, uint32_t cnt27, auto&& between) {
-            // The particle itself passes its own test (r = 0); d4 skips it. Empty
+            // The particle itself passes its own test (r = 0); dn skips it. Empty
             // slots are masked out with the (wave-uniform) slot masks.
             float4 xc[4];
             uint64_t occupied[4];
@@ -597,14 +632,8 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
             uint64_t m[4];
 #pragma unroll
             for (uint32_t r = 0; r < 4; ++r) m[r] = ((uint64_t)m8.w[2 * r + 1] << 32) | m8.w[2 * r];
-            walk_mask<kBatchDensity>(m, gt.start[wave][slot], i, d4);
+            walk_mask<kBatchDensity>(m, gt.start[wave][slot], i, dn);
             // Crowded buckets: their particles past the kSlots-th, a batch at a time.
-            // extraMask[i] bit k: extra particle k, enumerated bucket by bucket in
-            // visit order, is a neighbour; all ones when there are more than
-            // kExtraBits extras. The force pass reads it.
-            if (!crowded) return;
-            uint32_t k = 0;
-            uint64_t bits = 0ull;
             for (; crowded; crowded &= crowded - 1u) {
                 const uint32_t bi = (uint32_t)__builtin_ctz(crowded);
                 const uint32_t s0 = gt.start[wave][slot][bi], e = s0 + gt.cnt[wave][slot][bi];
@@ -612,80 +641,57 @@ __global__ __launch_bounds__(kBlock) void k_density_cw(const BlockDesc* __restri
                     uint32_t j[kBatchCrowded];
 #pragma unroll
                     for (uint32_t u = 0; u < kBatchCrowded; ++u) j[u] = jj + u < e ? jj + u : i;
-                    const uint32_t hits = d4(j);  // padded entries (j = i) never hit
-                    if (k < kExtraBits) bits |= (uint64_t)hits << k;
-                    k += min(kBatchCrowded, e - jj);
+                    dn(j);  // padded entries (j = i) are never kept
                 }
             }
-            extraMask[i] = k > kExtraBits ? ~0ull : bits;
+            flush();
         });
-    if (kStatic && ballot64(valid && (walls & ~kPoisonBit) != 0u) != 0ull && lane == 0u) heavy[blk] = 1u;
+    if (kStatic && ballot64(valid && (walls & ~kPoisonBit) != 0u) != 0ull && lane == 0u) out->heavy[blk] = 1u;
     if (!valid) return;
-    uint4* mrow = reinterpret_cast<uint4*>(mask + (size_t)i * kCwMaskWords);
-    mrow[0] = make_uint4(m8.w[0], m8.w[1], m8.w[2], m8.w[3]);
-    mrow[1] = make_uint4(m8.w[4], m8.w[5], m8.w[6], m8.w[7]);
+    out->nbrCount[i] = (walls & kPoisonBit) ? kListPoison : nc;
     if (kStatic) walk_walls<kBatchDensity>(walls & ~kPoisonBit, cx, cy, cz, S, stable, dWalls);
     // A poisoned particle's density is NaN, as the pile it skipped makes it.
     const float rho = (walls & kPoisonBit) ? __builtin_nanf("")
                                            : S.selfDensity + (sumDyn.s * S.mass + sumSt.s * S.mass) * S.C;
     // Only this lane writes pos[i].w; no lane of this pass reads any .w.
     reinterpret_cast<float*>(pos)[4ull * i + 3] = S.k * (rho - S.rho0);
-    if (kDebug) dens[i] = rho;
+    if (kDebug) out->dens[i] = rho;
     if (kStatic) {
         // w holds the sum of w2 with its sign bit set when a coincident pair
         // occurred (the sum itself is never negative); then wallZ holds its w1.
-        wallSum[i] = make_float4(wgx.s, wgy.s, wgz.s, wAny ? -wW2 : wW2);
-        if (wAny) wallZ[i] = make_float4(wZero, 0.f, 0.f, 0.f);
+        out->wallSum[i] = make_float4(wgx.s, wgy.s, wgz.s, wAny ? -wW2 : wW2);
+        if (wAny) out->wallZ[i] = make_float4(wZero, 0.f, 0.f, 0.f);
     }
 }
 
-// Eight waves per SIMD (at most 64 VGPRs) in FP32; the Kahan sums of the mixed
-// mode need more registers, so that mode settles for six.
+// Force pass of the cell-wave walk: each particle sums over the neighbour list
+// the density pass recorded (the same candidates in the same order as the
+// density pass kept them), so it needs no cell groups, tables or masks. A
+// particle whose list overflowed walks its 27 buckets by Morton runs instead.
 template <bool kStatic, bool kDebug, bool kTri, bool kMixed>
-__global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
+__global__ __launch_bounds__(kBlock) void k_force_cw(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
                                                      const SceneDev* __restrict__ scenes,
                                                      const float4* __restrict__ pos, const float4* __restrict__ vel,
                                                      const uint32_t* __restrict__ tab,
-                                                     const float4* __restrict__ spos,
-                                                     const uint32_t* __restrict__ stab,
-                                                     const uint32_t* __restrict__ order,
                                                      const float4* __restrict__ wallSum,
                                                      const float4* __restrict__ wallZ,
-                                                     const uint32_t* __restrict__ mask,
-                                                     const uint64_t* __restrict__ extraMask,
-                                                     const uint32_t* __restrict__ nanCnt, uint32_t* __restrict__ nanNext,
+                                                     const uint32_t* __restrict__ nbr,
+                                                     const uint32_t* __restrict__ nbrCount,
+                                                     uint32_t* __restrict__ nanNext,
                                                      const uint32_t* __restrict__ ids, const float* __restrict__ tris,
                                                      float dt, float4* __restrict__ posT, float4* __restrict__ velT,
                                                      uint32_t* __restrict__ counts, float4* __restrict__ accOut) {
-    __shared__ GroupTables gt;
-    const uint32_t tile = xcd_tile(blockIdx.x, numBlocks);
-    if (tile >= numBlocks) return;
-    const BlockDesc bd = blocks[order[tile]];
+    const uint32_t blk = xcd_tile(blockIdx.x, numBlocks);
+    if (blk >= numBlocks) return;
+    const BlockDesc bd = blocks[blk];
     const SceneDev& S = scenes[bd.scene];
     const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
-    const uint32_t wbegin = bd.begin + 64u * wave;
-    if (wbegin >= bd.end) return;
-    const uint32_t wcount = min(64u, bd.end - wbegin);
-    const uint32_t i = wbegin + lane;
-    const bool valid = lane < wcount;
-    const uint32_t iv = valid ? i : wbegin;
-    const float4 xi = ld16(pos, iv);
-    const float4 vi = ld16(vel, iv);
+    const uint32_t i = bd.begin + threadIdx.x;
+    if (i >= bd.end) return;  // the active lanes of a wave stay a prefix
+    const float4 xi = ld16(pos, i);
+    const float4 vi = ld16(vel, i);
     const float pi = xi.w;  // this step's pressure (density pass)
     const float h = S.h, h2 = S.h2;
-    const uint32_t cx = cell_coord(xi.x, h), cy = cell_coord(xi.y, h), cz = cell_coord(xi.z, h);
-    const uint32_t code = morton3(cx, cy, cz);
-    const uint32_t* table = tab + S.tBegin;
-    const uint32_t lo = S.pBegin, hi = S.pBegin + S.pCount;
-    uint64_t m[4];
-    {
-        const uint4* mr = reinterpret_cast<const uint4*>(mask + (size_t)iv * kCwMaskWords);
-        const uint4 a = mr[0], b = mr[1];
-        m[0] = ((uint64_t)a.y << 32) | a.x;
-        m[1] = ((uint64_t)a.w << 32) | a.z;
-        m[2] = ((uint64_t)b.y << 32) | b.x;
-        m[3] = ((uint64_t)b.w << 32) | b.z;
-    }
     const float rho = pi / S.k + S.rho0;  // SF:544
     const float absP = fabsf(pi);
     Acc<kMixed> fpx, fpy, fpz, fvx, fvy, fvz, fsx, fsy, fsz;
@@ -732,35 +738,29 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const Block
 #pragma unroll
         for (uint32_t u = 0; u < B; ++u) pair_at(j[u], xj[u], vj[u]);
     };
-    uint32_t noWalls = 0u;  // walls come from the density pass's sums; kPoisonBit
-    const uint64_t nanLanes = ballot64(valid && (xi.x != xi.x || xi.y != xi.y || xi.z != xi.z));
-    for_each_group<false>(
-        lane, wave, valid, code, cx, cy, cz, S, table, nullptr, nanCnt[bd.scene], nanLanes, lo, hi, gt, noWalls,
-        [&](uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, auto&& between) { between(); },
-        [&](uint32_t slot, uint32_t crowded) {
-            walk_mask<kBatchForce>(m, gt.start[wave][slot], i, pair4);
-            // Crowded buckets' extra particles: the density pass's neighbours
-            // among them when there are at most kExtraBits, else every one, a
-            // batch at a time.
-            if (!crowded) return;
-            uint64_t xb = extraMask[i];
-            const bool all = xb == ~0ull;
-            for (; crowded; crowded &= crowded - 1u) {
-                const uint32_t bi = (uint32_t)__builtin_ctz(crowded);
-                const uint32_t s0 = gt.start[wave][slot][bi] + kSlots, ne = gt.cnt[wave][slot][bi] - kSlots;
-                for (uint32_t base = 0; base < ne; base += 32u) {
-                    const uint32_t c = min(32u, ne - base);
-                    const uint32_t low = c < 32u ? (1u << c) - 1u : ~0u;
-                    uint32_t w = all ? low : (uint32_t)xb & low;
-                    xb >>= c;
-                    for (; w; w &= w - 1u) {  // one at a time: fewer registers
-                        const uint32_t j[1] = {s0 + base + (uint32_t)__builtin_ctz(w)};
-                        pair4(j);
-                    }
-                }
-            }
-        });
-    if (!valid) return;
+    const uint32_t n = nbrCount[i];
+    if (n <= kListCap) {
+        // The list, kBatchForce entries at a time; a wave reads one 256-byte row
+        // of indices per slot.
+        const uint32_t* list = nbr + (size_t)(blk * kWaves + wave) * (kListCap * 64u) + lane;
+        for (uint32_t s = 0; s < n; s += kBatchForce) {
+            uint32_t j[kBatchForce];
+#pragma unroll
+            for (uint32_t u = 0; u < kBatchForce; ++u) j[u] = s + u < n ? list[(s + u) << 6] : i;
+            pair4(j);
+        }
+    } else if (n != kListPoison) {
+        // More neighbours than the list holds: every candidate of the 27 buckets
+        // (SF:557-589), each bucket as often as the reference visits it.
+        const uint32_t lo = S.pBegin, hi = S.pBegin + S.pCount;
+        walk_runs(cell_coord(xi.x, h), cell_coord(xi.y, h), cell_coord(xi.z, h), S.M, S.magicM, tab + S.tBegin, lo,
+                  hi, [&](uint32_t b, uint32_t e) {
+                      for (uint32_t j = b; j < e; ++j) {
+                          const uint32_t jj[1] = {j};
+                          pair4(jj);
+                      }
+                  });
+    }
     if (kStatic) {
         // Walls (SF:590-623): |p_i| times the density pass's geometric sums.
         const float4 ws = ld16(wallSum, i);
@@ -795,9 +795,10 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw(const Block
             Fz += f.z;
         }
     }
-    // A poisoned particle's acceleration is NaN, as the pile it skipped makes it.
+    // A poisoned particle's acceleration is NaN, as the pile the density pass
+    // skipped makes it.
     const float nan = __builtin_nanf("");
-    const bool poisoned = (noWalls & kPoisonBit) != 0u;
+    const bool poisoned = n == kListPoison;
     const float ax = poisoned ? nan : Fx / rho + S.gx, ay = poisoned ? nan : Fy / rho + S.gy,
                 az = poisoned ? nan : Fz / rho + S.gz;
     if (kDebug) accOut[i] = make_float4(ax, ay, az, 0.f);

@@ -73,7 +73,9 @@ struct sph_engine {
     float* dens = nullptr;
     float4* acc = nullptr;
     uint32_t* mask = nullptr;       // candidate mask from the density pass (run, tile and cell-wave walks)
-    uint64_t* extraMask = nullptr;  // cell-wave: neighbours among crowded buckets' extra particles
+    uint32_t* nbr = nullptr;        // cell-wave: neighbour lists (kListCap entries per particle)
+    uint32_t* nbrCount = nullptr;   // cell-wave: list lengths
+    DensityOut* densOut = nullptr;  // cell-wave density pass: its output pointers (device copy)
     uint32_t* nanCnt = nullptr;     // 2 x nScenes: NaN particles in bucket 0, per parity
     double* consPartial = nullptr;
     double* consOut = nullptr;
@@ -172,7 +174,9 @@ void release(sph_engine* h) {
     dfree(h->dens);
     dfree(h->acc);
     dfree(h->mask);
-    dfree(h->extraMask);
+    dfree(h->nbr);
+    dfree(h->nbrCount);
+    dfree(h->densOut);
     dfree(h->nanCnt);
     dfree(h->consPartial);
     dfree(h->consOut);
@@ -304,11 +308,11 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     if (walk == Walk::CellWave && mixed)                                                                       \
         hipLaunchKernelGGL((k_density_cw<S, D, true>), gridX, block, 0, h->stream, h->dBlocks,            \
                            h->numBlocks, h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->wallMap,        \
-                           h->tileOrder, h->heavy, h->wallSum, h->wallZ, h->mask, h->extraMask, nanNow, h->dens); \
+                           h->tileOrder, h->nbr, nanNow, h->densOut);                                      \
     else if (walk == Walk::CellWave)                                                                           \
         hipLaunchKernelGGL((k_density_cw<S, D, false>), gridX, block, 0, h->stream, h->dBlocks,            \
                            h->numBlocks, h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->wallMap,        \
-                           h->tileOrder, h->heavy, h->wallSum, h->wallZ, h->mask, h->extraMask, nanNow, h->dens); \
+                           h->tileOrder, h->nbr, nanNow, h->densOut);                                      \
     else if (walk == Walk::Tiles)                                                                              \
         hipLaunchKernelGGL((k_density_tile<S, D>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,       \
                            h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->mask, h->dens);                 \
@@ -352,15 +356,13 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
 #define SPH_FORCE(S, D, T)                                                                                        \
     do {                                                                                                          \
     if (walk == Walk::CellWave && mixed)                                                                          \
-        hipLaunchKernelGGL((k_force_cw<S, D, T, true>), gridX, block, 0, h->stream, h->dBlocks,                \
-                           h->numBlocks, h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->tileOrder,  \
-                           h->wallSum, h->wallZ, h->mask, h->extraMask, nanNow, nanNext, h->ids[c], h->tris, dt, \
-                           h->posT, h->velT, h->tab[c ^ 1], h->acc);                                              \
+        hipLaunchKernelGGL((k_force_cw<S, D, T, true>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,    \
+                           h->dScenes, h->pos, h->vel, h->tab[c], h->wallSum, h->wallZ, h->nbr, h->nbrCount,      \
+                           nanNext, h->ids[c], h->tris, dt, h->posT, h->velT, h->tab[c ^ 1], h->acc);             \
     else if (walk == Walk::CellWave)                                                                              \
-        hipLaunchKernelGGL((k_force_cw<S, D, T, false>), gridX, block, 0, h->stream, h->dBlocks,                \
-                           h->numBlocks, h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->tileOrder,  \
-                           h->wallSum, h->wallZ, h->mask, h->extraMask, nanNow, nanNext, h->ids[c], h->tris, dt, \
-                           h->posT, h->velT, h->tab[c ^ 1], h->acc);                                              \
+        hipLaunchKernelGGL((k_force_cw<S, D, T, false>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,   \
+                           h->dScenes, h->pos, h->vel, h->tab[c], h->wallSum, h->wallZ, h->nbr, h->nbrCount,      \
+                           nanNext, h->ids[c], h->tris, dt, h->posT, h->velT, h->tab[c ^ 1], h->acc);             \
     else if (walk == Walk::Tiles)                                                                                 \
         hipLaunchKernelGGL((k_force_tile<S, D, T>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,         \
                            h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->mask, h->ids[c], h->tris,  \
@@ -673,7 +675,8 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
                   "every mask format uses 8 words per particle");
     if (!(flags & SPH_FLAG_CELL_WALK)) {
         SPH_ALLOC(h->mask, kMaskWords * std::max<uint64_t>(N, 1));
-        SPH_ALLOC(h->extraMask, std::max<uint64_t>(N, 1));
+        SPH_ALLOC(h->nbr, (uint64_t)std::max<uint32_t>(h->numBlocks, 1) * kWaves * kListCap * 64u);
+        SPH_ALLOC(h->nbrCount, std::max<uint64_t>(N, 1));
     }
     if (flags & SPH_FLAG_RECORD_DEBUG) {
         SPH_ALLOC(h->dens, std::max<uint64_t>(N, 1));
@@ -704,6 +707,11 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
         for (size_t t = 0; t < identity.size(); ++t) identity[t] = (uint32_t)t;
         SPH_HIP(h, hipMemcpy(h->tileOrder, identity.data(), sizeof(uint32_t) * identity.size(), hipMemcpyHostToDevice));
         SPH_HIP(h, hipMemset(h->heavy, 0, sizeof(uint32_t) * identity.size()));
+    }
+    if (h->nbrCount) {
+        const DensityOut o{h->heavy, h->nbrCount, h->wallSum, h->wallZ, h->dens};
+        if ((rc = dalloc(h, &h->densOut, 1))) return rc;
+        SPH_HIP(h, hipMemcpy(h->densOut, &o, sizeof(o), hipMemcpyHostToDevice));
     }
     if (Ns) {
         SPH_HIP(h, hipMemcpy(h->spos, hSpos.data(), sizeof(float4) * Ns, hipMemcpyHostToDevice));
