@@ -53,22 +53,21 @@ struct Mask8 {
 // with the lane select in m0, since the data operand already uses the one scalar
 // operand a VALU instruction may read; the s_nop covers the wait states between
 // writing m0 and reading it as a lane select).
+// Round 3 holds buckets 24..26 on lanes 0..23 only, so the high word of its
+// ballot is always zero and w[7] is never written (it stays zero).
 __device__ inline void writelane4(Mask8& m, const uint64_t (&b)[4], uint32_t t) {
     asm volatile(
         "s_nop 4\n\t"
-        "v_writelane_b32 %0, %8, m0\n\t"
-        "v_writelane_b32 %1, %9, m0\n\t"
-        "v_writelane_b32 %2, %10, m0\n\t"
-        "v_writelane_b32 %3, %11, m0\n\t"
-        "v_writelane_b32 %4, %12, m0\n\t"
-        "v_writelane_b32 %5, %13, m0\n\t"
-        "v_writelane_b32 %6, %14, m0\n\t"
-        "v_writelane_b32 %7, %15, m0"
-        : "+v"(m.w[0]), "+v"(m.w[1]), "+v"(m.w[2]), "+v"(m.w[3]), "+v"(m.w[4]), "+v"(m.w[5]), "+v"(m.w[6]),
-          "+v"(m.w[7])
+        "v_writelane_b32 %0, %7, m0\n\t"
+        "v_writelane_b32 %1, %8, m0\n\t"
+        "v_writelane_b32 %2, %9, m0\n\t"
+        "v_writelane_b32 %3, %10, m0\n\t"
+        "v_writelane_b32 %4, %11, m0\n\t"
+        "v_writelane_b32 %5, %12, m0\n\t"
+        "v_writelane_b32 %6, %13, m0"
+        : "+v"(m.w[0]), "+v"(m.w[1]), "+v"(m.w[2]), "+v"(m.w[3]), "+v"(m.w[4]), "+v"(m.w[5]), "+v"(m.w[6])
         : "s"((uint32_t)b[0]), "s"((uint32_t)(b[0] >> 32)), "s"((uint32_t)b[1]), "s"((uint32_t)(b[1] >> 32)),
-          "s"((uint32_t)b[2]), "s"((uint32_t)(b[2] >> 32)), "s"((uint32_t)b[3]), "s"((uint32_t)(b[3] >> 32)),
-          "{m0}"(t));
+          "s"((uint32_t)b[2]), "s"((uint32_t)(b[2] >> 32)), "s"((uint32_t)b[3]), "{m0}"(t));
 }
 
 // Record j of an array of 16-byte records through a 32-bit byte offset (the
@@ -218,18 +217,11 @@ struct GroupFetch {
 // Bit of myWalls (for_each_group) that marks a poisoned particle.
 constexpr uint32_t kPoisonBit = 0x80000000u;
 
-// Neighbour cell `lane` (lanes past 26 repeat cell 26) of the group whose first
-// lane is g0, in the reference's x-y-z visit order (SF:397-404).
-__device__ inline void neighbour_cell(uint32_t lane, uint32_t g0, uint32_t cx, uint32_t cy, uint32_t cz,
-                                      uint32_t& nx, uint32_t& ny, uint32_t& nz) {
-    const uint32_t gx = (uint32_t)__builtin_amdgcn_readlane((int)cx, (int)g0);
-    const uint32_t gy = (uint32_t)__builtin_amdgcn_readlane((int)cy, (int)g0);
-    const uint32_t gz = (uint32_t)__builtin_amdgcn_readlane((int)cz, (int)g0);
-    const uint32_t n = lane < 27u ? lane : 26u;
-    nx = gx + n / 9u - 1u;
-    ny = gy + (n / 3u) % 3u - 1u;
-    nz = gz + n % 3u - 1u;
-}
+// Neighbour cell `lane` of a group is cell `lane` of the reference's x-y-z
+// visit order (SF:397-404) around the group's cell; lanes past 26 repeat cell
+// 26. `noff` holds the lane's dilated offsets (neighbour_offsets), and the
+// cell's Morton code is neighbour_code of the group's code.
+__device__ inline uint32_t lane_offsets(uint32_t lane) { return neighbour_offsets(lane < 27u ? lane : 26u); }
 
 // NaN particles. A particle with a NaN coordinate has the NaN distance to every
 // candidate, which the reference keeps (within_h2), so every particle that
@@ -243,17 +235,16 @@ __device__ inline void neighbour_cell(uint32_t lane, uint32_t g0, uint32_t cx, u
 // empty) instead of walking the pile, and the passes write NaN results.
 // NaN particles outside bucket 0 are walked like any other.
 template <bool kStatic>
-__device__ inline GroupFetch fetch_group(uint64_t& heads, uint64_t stops, uint32_t lane, uint32_t cx, uint32_t cy,
-                                         uint32_t cz, const SceneDev& S, const uint32_t* __restrict__ table,
+__device__ inline GroupFetch fetch_group(uint64_t& heads, uint64_t stops, uint32_t lane, uint32_t code,
+                                         uint32_t noff, const SceneDev& S, const uint32_t* __restrict__ table,
                                          const uint32_t* __restrict__ wallMap, uint32_t nan0, uint64_t nanLanes) {
     GroupFetch f;
     f.g0 = (uint32_t)__builtin_ctzll(heads);
     heads &= heads - 1ull;
     const uint64_t above = f.g0 < 63u ? (stops & ~((2ull << f.g0) - 1ull)) : 0ull;
     f.g1 = above ? (uint32_t)__builtin_ctzll(above) : 64u;
-    uint32_t nx, ny, nz;
-    neighbour_cell(lane, f.g0, cx, cy, cz, nx, ny, nz);
-    const uint32_t bk = fastmod(morton3(nx, ny, nz), S.magicM, S.M);
+    const uint32_t ncode = neighbour_code((uint32_t)__builtin_amdgcn_readlane((int)code, (int)f.g0), noff);
+    const uint32_t bk = fastmod(ncode, S.magicM, S.M);
     const uint64_t groupLanes = (f.g1 < 64u ? (1ull << f.g1) : 0ull) - (1ull << f.g0);
     f.nanGroup = (nanLanes & groupLanes) != 0ull;
     f.poison = nan0 != 0u && (nan0 >= 2u || !f.nanGroup) && ballot64(lane < 27u && bk == 0u) != 0ull;
@@ -262,7 +253,7 @@ __device__ inline GroupFetch fetch_group(uint64_t& heads, uint64_t stops, uint32
     // A dropped bucket 0 reads entry 1 twice: an empty range.
     f.tb = table[f.poison && bk == 0u ? 1u : bk];
     f.te = table[bk + 1];
-    f.wword = kStatic ? wallMap[S.wmBegin + ((cell_hash(nx, ny, nz) >> 5) & S.wmMask)] : 0u;
+    f.wword = kStatic ? wallMap[S.wmBegin + ((wall_hash(ncode) >> 5) & S.wmMask)] : 0u;
     return f;
 }
 
@@ -276,15 +267,14 @@ __device__ inline GroupFetch fetch_group(uint64_t& heads, uint64_t stops, uint32
 // sum. The exception is a particle with a NaN coordinate, whose distances are
 // NaN and kept; a group holding one (`all`) walks the static buckets of all 27
 // cells.
-__device__ inline uint32_t wall_visits(uint32_t wword, uint32_t lane, uint32_t g0, uint32_t cx, uint32_t cy,
-                                       uint32_t cz, const SceneDev& S, bool all) {
+__device__ inline uint32_t wall_visits(uint32_t wword, uint32_t lane, uint32_t g0, uint32_t code, uint32_t noff,
+                                       const SceneDev& S, bool all) {
     if (S.sCount == 0u) return 0u;
-    uint32_t nx, ny, nz;
-    neighbour_cell(lane, g0, cx, cy, cz, nx, ny, nz);
-    const bool hit = lane < 27u && (all || ((wword >> (cell_hash(nx, ny, nz) & 31u)) & 1u));
+    const uint32_t ncode = neighbour_code((uint32_t)__builtin_amdgcn_readlane((int)code, (int)g0), noff);
+    const bool hit = lane < 27u && (all || ((wword >> (wall_hash(ncode) & 31u)) & 1u));
     uint32_t hits = (uint32_t)ballot64(hit);
     if (hits == 0u) return 0u;
-    const uint32_t sb = fastmod(morton3(nx, ny, nz), S.magicMs, S.Ms);
+    const uint32_t sb = fastmod(ncode, S.magicMs, S.Ms);
     uint32_t visits = 0u;
     for (; hits; hits &= hits - 1u) {
         const uint32_t sbn = (uint32_t)__builtin_amdgcn_readlane((int)sb, (int)__builtin_ctz(hits));
@@ -306,8 +296,8 @@ __device__ inline uint32_t wall_visits(uint32_t wword, uint32_t lane, uint32_t g
 // kPoisonBit alone for a poisoned group (fetch_group), which walks no walls.
 // nanLanes: the lanes whose particle has a NaN coordinate.
 template <bool kStatic, class P1, class P2>
-__device__ inline void for_each_group(uint32_t lane, uint32_t wave, bool valid, uint32_t code, uint32_t cx,
-                                      uint32_t cy, uint32_t cz, const SceneDev& S, const uint32_t* __restrict__ table,
+__device__ inline void for_each_group(uint32_t lane, uint32_t wave, bool valid, uint32_t code, uint32_t noff,
+                                      const SceneDev& S, const uint32_t* __restrict__ table,
                                       const uint32_t* __restrict__ wallMap, uint32_t nan0, uint64_t nanLanes,
                                       uint32_t lo, uint32_t hi, GroupTables& gt, uint32_t& myWalls, P1&& phase1,
                                       P2&& phase2) {
@@ -316,7 +306,7 @@ __device__ inline void for_each_group(uint32_t lane, uint32_t wave, bool valid, 
     const uint64_t stops = heads | ~ballot64(valid);  // a group ends at the next head or invalid lane
     while (heads) {
         uint32_t mySlot = 0xFFFFFFFFu, myCrowded = 0u;
-        GroupFetch cur = fetch_group<kStatic>(heads, stops, lane, cx, cy, cz, S, table, wallMap, nan0, nanLanes);
+        GroupFetch cur = fetch_group<kStatic>(heads, stops, lane, code, noff, S, table, wallMap, nan0, nanLanes);
         for (uint32_t slot = 0;; ++slot) {
             const uint32_t s0 = cur.tb > lo ? cur.tb : lo, e0 = cur.te < hi ? cur.te : hi;
             const uint32_t start27 = s0, cnt27 = lane < 27u && e0 > s0 ? e0 - s0 : 0u;  // empty past lane 26
@@ -326,7 +316,7 @@ __device__ inline void for_each_group(uint32_t lane, uint32_t wave, bool valid, 
             }
             const uint32_t crowded = (uint32_t)ballot64(cnt27 > kSlots);  // buckets 0..26
             const uint32_t walls =
-                kStatic && !cur.poison ? wall_visits(cur.wword, lane, cur.g0, cx, cy, cz, S, cur.nanGroup) : 0u;
+                kStatic && !cur.poison ? wall_visits(cur.wword, lane, cur.g0, code, noff, S, cur.nanGroup) : 0u;
             if (lane >= cur.g0 && lane < cur.g1) {
                 mySlot = slot;
                 myCrowded = crowded;
@@ -337,7 +327,7 @@ __device__ inline void for_each_group(uint32_t lane, uint32_t wave, bool valid, 
             uint64_t rest = more ? heads : 1ull;  // at the batch end: a harmless read of group 0's table
             phase1(slot, cur.g0, cur.g1, start27, cnt27,
                    [&]() {
-                       nxt = fetch_group<kStatic>(rest, stops, lane, cx, cy, cz, S, table, wallMap, nan0, nanLanes);
+                       nxt = fetch_group<kStatic>(rest, stops, lane, code, noff, S, table, wallMap, nan0, nanLanes);
                    });
             if (!more) break;
             heads = rest;
@@ -566,7 +556,7 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : SPH_CW_DENSITY_WAVES) void k_d
     uint32_t walls = 0u;  // static buckets this lane walks, or kPoisonBit
     const uint64_t nanLanes = ballot64(valid && (xi.x != xi.x || xi.y != xi.y || xi.z != xi.z));
     for_each_group<kStatic>(
-        lane, wave, valid, code, cx, cy, cz, S, table, wallMap, nanCnt[bd.scene], nanLanes, lo, hi, gt, walls,
+        lane, wave, valid, code, lane_offsets(lane), S, table, wallMap, nanCnt[bd.scene], nanLanes, lo, hi, gt, walls,
         [&](uint32_t, uint32_t g0, uint32_t g1, uint32_t start27, uint32_t cnt27, auto&& between) {
             // The particle itself passes its own test (r = 0); dn skips it. Empty
             // slots are masked out with the (wave-uniform) slot masks.

@@ -62,7 +62,7 @@ struct sph_engine {
     uint32_t* ticket = nullptr;
     float4* spos = nullptr;
     uint32_t* stab = nullptr;
-    uint32_t* wallMap = nullptr;  // per scene: bit cell_hash(c) set for every cell c holding walls
+    uint32_t* wallMap = nullptr;  // per scene: bit wall_hash(Morton code of c) set for every cell c holding walls
     uint32_t* tileOrder = nullptr;  // cell-wave launch order (k_tile_order), gridX entries
     uint32_t* heavy = nullptr;      // per block: particles near walls in the last density pass
     float4* wallSum = nullptr;      // cell-wave: the walls' geometric sums of each particle, for the force pass
@@ -590,7 +590,7 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
             hWallMap.resize(hWallMap.size() + words, 0u);
             for (uint64_t i = 0; i < sn; ++i) {
                 const float* q = static_pos + 3 * (s0 + i);
-                const uint32_t hc = cell_hash(cell_coord(q[0], S.h), cell_coord(q[1], S.h), cell_coord(q[2], S.h));
+                const uint32_t hc = wall_hash(morton3(cell_coord(q[0], S.h), cell_coord(q[1], S.h), cell_coord(q[2], S.h)));
                 hWallMap[S.wmBegin + ((hc >> 5) & S.wmMask)] |= 1u << (hc & 31u);
             }
         }

@@ -16,12 +16,11 @@ __host__ __device__ inline uint32_t cell_coord(float x, float h) {
     return (uint32_t)(int32_t)floorf(x / h);
 }
 
-// Hash of a full cell coordinate (the 32-bit wrap of the signed cell index per
-// axis), for the wall-cell bitmap: a multiplicative mix of the three
-// coordinates followed by xor-shift-multiply rounds.
-__host__ __device__ inline uint32_t cell_hash(uint32_t cx, uint32_t cy, uint32_t cz) {
-    uint32_t h = cx * 0x9E3779B1u ^ cy * 0x85EBCA77u ^ cz * 0xC2B2AE3Du;
-    h ^= h >> 15;
+// Hash of a cell's Morton code for the wall-cell bitmap (xor-shift-multiply
+// rounds). Cells whose codes alias (1024 cells apart on an axis) share a bit,
+// which only makes the bitmap's filter looser.
+__host__ __device__ inline uint32_t wall_hash(uint32_t code) {
+    uint32_t h = code ^ (code >> 15);
     h *= 0x2C1B3C6Du;
     h ^= h >> 12;
     h *= 0x297A2D39u;
@@ -42,6 +41,25 @@ __host__ __device__ inline uint32_t spread3(uint32_t v) {
 // SF:88-97, SF:103-106: 30-bit Morton code, x on bit 3k, y on 3k+1, z on 3k+2.
 __host__ __device__ inline uint32_t morton3(uint32_t cx, uint32_t cy, uint32_t cz) {
     return spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2);
+}
+
+// Neighbour n = 9a + 3b + c (a, b, c in 0..2) of a cell is the cell offset by
+// (a - 1, b - 1, c - 1); its dilated offsets (a, b, c spread as morton3 does).
+__host__ __device__ inline uint32_t neighbour_offsets(uint32_t n) {
+    return morton3(n / 9u, (n / 3u) % 3u, n % 3u);
+}
+
+// Morton code of the neighbour with dilated offsets `off` of the cell whose
+// code is c, equal to morton3 of the offset coordinates (each wrapping at 1024
+// as morton3 does). Per axis in dilated arithmetic: c's coordinate minus one
+// (the borrow runs through the zero gaps), the gaps filled with ones so the
+// carry of adding the offset runs through them, then the axis's bits.
+__host__ __device__ inline uint32_t neighbour_code(uint32_t c, uint32_t off) {
+    constexpr uint32_t mx = 0x09249249u, my = mx << 1, mz = mx << 2;
+    const uint32_t bx = (((c & mx) - 1u) & mx) | ~mx;
+    const uint32_t by = (((c & my) - 2u) & my) | ~my;
+    const uint32_t bz = (((c & mz) - 4u) & mz) | ~mz;
+    return ((bx + (off & mx)) & mx) | ((by + (off & my)) & my) | ((bz + (off & mz)) & mz);
 }
 
 // Remainder by a runtime divisor d without a hardware divide: Lemire, Kaser and

@@ -1,7 +1,8 @@
 // Host-side check of the device neighbour walk (sphfluidsimulation_amd/csrc/sph_math.hpp):
 // for random cells and bucket counts M, the buckets walk_runs visits (with
 // multiplicity) are exactly the buckets of the reference's 27-cell loop
-// (SPHFunctions.cpp:397-404), and fastmod equals %. Exit status 0 on success.
+// (SPHFunctions.cpp:397-404), fastmod equals %, and the dilated-arithmetic
+// neighbour codes equal morton3 of the neighbour cells. Exit status 0 on success.
 #include <algorithm>
 #include <cstdio>
 #include <random>
@@ -22,6 +23,22 @@ int main() {
         for (int k = 0; k < 8; ++k) {
             const uint32_t a = (uint32_t)rng();
             bad += fastmod(a, m, d) != a % d;
+            ++checked;
+        }
+    }
+    // The cell-wave passes' neighbour codes: dilated arithmetic on the cell's
+    // own code against morton3 of the offset coordinates, in the 27-cell order.
+    for (int t = 0; t < 200000; ++t) {
+        uint32_t c[3];
+        for (int a = 0; a < 3; ++a) {
+            const int kind = (int)(rng() % 3);
+            c[a] = kind == 0 ? (uint32_t)(rng() % 4) - 2u : kind == 1 ? 1022u + (uint32_t)(rng() % 4)
+                                                                      : (uint32_t)rng();
+        }
+        const uint32_t self = morton3(c[0], c[1], c[2]);
+        for (uint32_t n = 0; n < 27; ++n) {
+            const uint32_t want = morton3(c[0] + n / 9u - 1u, c[1] + (n / 3u) % 3u - 1u, c[2] + n % 3u - 1u);
+            bad += neighbour_code(self, neighbour_offsets(n)) != want;
             ++checked;
         }
     }
