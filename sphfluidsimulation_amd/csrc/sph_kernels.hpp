@@ -678,30 +678,44 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* __restrict__ da
         rowOff[r] = agg;
         agg += rowTotal;
     }
-    if (tid == 0) {
+    if (tid < 64u) {
+        // Look-back by the first wave, 64 predecessors at a time: lane l reads
+        // the status of tile t - 1 - l; the lanes up to the nearest inclusive
+        // prefix must all be published, and their values sum to this tile's
+        // exclusive prefix (tiles before the first count as an inclusive 0).
         uint32_t prefix = 0;
         if (tile == 0) {
-            __hip_atomic_store(&status[0], kStatIncl | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (tid == 0) __hip_atomic_store(&status[0], kStatIncl | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            __hip_atomic_store(&status[tile], kStatAgg | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            uint32_t j = tile - 1u;
+            if (tid == 0) __hip_atomic_store(&status[tile], kStatAgg | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int32_t j0 = (int32_t)tile - 1;
             for (;;) {
+                const int32_t j = j0 - (int32_t)tid;
                 const unsigned long long st =
-                    __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const unsigned long long flag = st >> 32;
-                if (flag == 0ull) {
+                    j >= 0 ? __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : kStatIncl;
+                const uint32_t flag = (uint32_t)(st >> 32);
+                const uint64_t incl = __ballot(flag == 2u);
+                const uint32_t first = incl ? (uint32_t)__builtin_ctzll(incl) : 63u;  // last lane that counts
+                const uint64_t upto = first == 63u ? ~0ull : (2ull << first) - 1ull;
+                if (__ballot(flag == 0u) & upto) {
                     __builtin_amdgcn_s_sleep(1);
                     continue;
                 }
-                prefix += (uint32_t)st;
-                if (flag == 2ull) break;
-                --j;
+                uint32_t val = tid <= first ? (uint32_t)st : 0u;
+#pragma unroll
+                for (int d = 32; d >= 1; d >>= 1) val += (uint32_t)__shfl_xor((int)val, d);
+                prefix += val;
+                if (incl) break;
+                j0 -= 64;
             }
-            __hip_atomic_store(&status[tile], kStatIncl | (uint32_t)(prefix + agg), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+            if (tid == 0)
+                __hip_atomic_store(&status[tile], kStatIncl | (uint32_t)(prefix + agg), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         }
-        s_prefix = prefix;
-        status_clear[tile] = 0ull;
+        if (tid == 0) {
+            s_prefix = prefix;
+            status_clear[tile] = 0ull;
+        }
     }
     __syncthreads();
     const uint32_t prefix = s_prefix;
