@@ -615,8 +615,11 @@ __global__ __launch_bounds__(kBlock) void k_force_runs(const BlockDesc* __restri
 // words of `clear` (the per-scene counts the next force pass accumulates), so
 // no memset is needed between steps and the launch can live in a hipGraph.
 constexpr int kScanThreads = 256;
-constexpr int kScanRows = 4;
-constexpr uint32_t kScanTile = kScanThreads * 4u * kScanRows;  // 4096 entries per tile
+#ifndef SPH_SCAN_ROWS
+#define SPH_SCAN_ROWS 8
+#endif
+constexpr int kScanRows = SPH_SCAN_ROWS;
+constexpr uint32_t kScanTile = kScanThreads * 4u * kScanRows;  // 8192 entries per tile
 constexpr unsigned long long kStatAgg = 1ull << 32;
 constexpr unsigned long long kStatIncl = 2ull << 32;
 
