@@ -109,6 +109,12 @@ constexpr uint32_t kBatchDensity = SPH_CW_BATCH_DENSITY;
 #endif
 constexpr uint32_t kBatchCrowded = SPH_CW_BATCH_CROWDED;
 constexpr uint32_t kBatchForce = SPH_CW_BATCH_FORCE;
+// Density pass, walls: walked after the cell groups, when the groups' registers
+// are free, so more loads can be in flight.
+#ifndef SPH_CW_BATCH_WALL
+#define SPH_CW_BATCH_WALL 8
+#endif
+constexpr uint32_t kBatchWall = SPH_CW_BATCH_WALL;
 // Neighbour lists. The density pass records, for the force pass, every
 // candidate it keeps (within_h2, j != i) in visit order: up to kListCap per
 // particle, slot n of the particle on lane l of wave w (workgroup b, wave
@@ -529,11 +535,11 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : SPH_CW_DENSITY_WAVES) void k_d
 
     // Static neighbours (SF:455-496).
     auto dWalls = [&](uint32_t j0, uint32_t e) {
-        float4 xj[kBatchDensity];
+        float4 xj[kBatchWall];
 #pragma unroll
-        for (uint32_t u = 0; u < kBatchDensity; ++u) xj[u] = ld16(spos, j0 + u < e ? j0 + u : j0);
+        for (uint32_t u = 0; u < kBatchWall; ++u) xj[u] = ld16(spos, j0 + u < e ? j0 + u : j0);
 #pragma unroll
-        for (uint32_t u = 0; u < kBatchDensity; ++u) {
+        for (uint32_t u = 0; u < kBatchWall; ++u) {
             const float dx = xj[u].x - xi.x, dy = xj[u].y - xi.y, dz = xj[u].z - xi.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
             const float r = __builtin_amdgcn_sqrtf(r2);
@@ -639,7 +645,7 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : SPH_CW_DENSITY_WAVES) void k_d
     if (kStatic && ballot64(valid && (walls & ~kPoisonBit) != 0u) != 0ull && lane == 0u) out->heavy[blk] = 1u;
     if (!valid) return;
     out->nbrCount[i] = (walls & kPoisonBit) ? kListPoison : nc;
-    if (kStatic) walk_walls<kBatchDensity>(walls & ~kPoisonBit, cx, cy, cz, S, stable, dWalls);
+    if (kStatic) walk_walls<kBatchWall>(walls & ~kPoisonBit, cx, cy, cz, S, stable, dWalls);
     // A poisoned particle's density is NaN, as the pile it skipped makes it.
     const float rho = (walls & kPoisonBit) ? __builtin_nanf("")
                                            : S.selfDensity + (sumDyn.s * S.mass + sumSt.s * S.mass) * S.C;
