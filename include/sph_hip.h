@@ -50,8 +50,13 @@ extern "C" {
 #define SPH_FLAG_RUN_WALK 0x10u     /* neighbour passes read candidates from HBM in merged Morton runs */
 #define SPH_FLAG_TILE_WALK 0x20u    /* neighbour passes stage each workgroup's buckets in LDS.
                                        Default (no walk flag): cell-group waves, where a wave tests
-                                       the candidates of a cell against all of the cell's particles
-                                       and keeps a per-particle neighbour mask for the force pass */
+                                       the candidates of a cell against all of the cell's particles;
+                                       the density pass hands the force pass each particle's neighbour
+                                       list (up to SPH_CW_LIST_MAX particles in the engine) or its
+                                       neighbour mask (more particles, or either flag below) */
+#define SPH_FLAG_CW_LISTS 0x40u     /* default walk: neighbour lists whatever the particle count */
+#define SPH_FLAG_CW_MASKS 0x80u     /* default walk: neighbour masks whatever the particle count */
+#define SPH_CW_LIST_MAX 1572864u    /* particles up to which the default walk uses lists */
 
 /* sph_validate result bits (the invariants of SimulationEngine.h:115-282). */
 #define SPH_V_NONFINITE 0x1u        /* NaN or Inf in position, velocity or pressure */

@@ -27,6 +27,9 @@ SPH_FLAG_FP16_OFFSETS = 0x4
 SPH_FLAG_CELL_WALK = 0x8
 SPH_FLAG_RUN_WALK = 0x10
 SPH_FLAG_TILE_WALK = 0x20
+SPH_FLAG_CW_LISTS = 0x40
+SPH_FLAG_CW_MASKS = 0x80
+SPH_CW_LIST_MAX = 1572864
 
 SPH_V_NONFINITE = 0x1
 SPH_V_KEY = 0x2

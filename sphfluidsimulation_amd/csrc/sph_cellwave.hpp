@@ -109,6 +109,12 @@ constexpr uint32_t kBatchDensity = SPH_CW_BATCH_DENSITY;
 #endif
 constexpr uint32_t kBatchCrowded = SPH_CW_BATCH_CROWDED;
 constexpr uint32_t kBatchForce = SPH_CW_BATCH_FORCE;
+// Force pass over masks: it also holds the cell groups and the masks in
+// registers, so it takes two candidates at a time.
+constexpr uint32_t kBatchForceMask = 2;
+// Crowded buckets, mask mode: the density pass records which of a particle's
+// first kExtraBits extra candidates are neighbours, for the force pass.
+constexpr uint32_t kExtraBits = 64;
 // Density pass, walls: walked after the cell groups, when the groups' registers
 // are free, so more loads can be in flight.
 #ifndef SPH_CW_BATCH_WALL
@@ -433,14 +439,18 @@ __global__ __launch_bounds__(256) void k_tile_order(uint32_t* __restrict__ heavy
 // Where the density pass writes its per-particle results (read once, at the
 // end of the pass, so these pointers hold no registers during the walk).
 struct DensityOut {
-    uint32_t* heavy;     // workgroups with walls in reach
-    uint32_t* nbrCount;  // neighbour list lengths
-    float4* wallSum;     // walls' geometric sums
-    float4* wallZ;       // coincident walls' w1
-    float* dens;         // density (debug recording)
+    uint32_t* heavy;      // workgroups with walls in reach
+    uint32_t* nbrCount;   // neighbour list lengths (list mode)
+    uint32_t* mask;       // neighbour masks (mask mode)
+    uint64_t* extraMask;  // crowded buckets' neighbours (mask mode)
+    float4* wallSum;      // walls' geometric sums
+    float4* wallZ;        // coincident walls' w1
+    float* dens;          // density (debug recording)
 };
 
-template <bool kStatic, bool kDebug, bool kMixed>
+// kList: record neighbour lists for k_force_cw; else the neighbour masks and
+// the crowded buckets' record for k_force_cw_mask.
+template <bool kStatic, bool kDebug, bool kMixed, bool kList>
 #ifndef SPH_CW_DENSITY_WAVES
 #define SPH_CW_DENSITY_WAVES 7
 #endif
@@ -503,6 +513,7 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : SPH_CW_DENSITY_WAVES) void k_d
     uint32_t pend[4] = {0u, 0u, 0u, 0u};  // the last batch, not yet appended
     uint32_t pendHits = 0u;               // its kept candidates (bit u)
     auto flush = [&]() {
+        if constexpr (!kList) return;
 #pragma unroll
         for (uint32_t u = 0; u < 4; ++u) {
             const bool hit = (pendHits >> u) & 1u;
@@ -511,7 +522,7 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : SPH_CW_DENSITY_WAVES) void k_d
         }
         pendHits = 0u;
     };
-    auto dn = [&](const auto& j) {
+    auto dn = [&](const auto& j) -> uint32_t {
         constexpr uint32_t B = sizeof(j) / sizeof(j[0]);
         static_assert(B <= 4, "at most four pending candidates");
         float4 xj[B];
@@ -528,9 +539,10 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : SPH_CW_DENSITY_WAVES) void k_d
             const bool ok = within_h2(r2, h2) && j[u] != i;
             sumDyn.add((ok && inside_h(r, h)) ? q * q * q : 0.f);
             hits |= ok ? 1u << u : 0u;
-            pend[u] = j[u];
+            if constexpr (kList) pend[u] = j[u];
         }
-        pendHits = hits;
+        if constexpr (kList) pendHits = hits;
+        return hits;
     };
 
     // Static neighbours (SF:455-496).
@@ -630,6 +642,12 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : SPH_CW_DENSITY_WAVES) void k_d
             for (uint32_t r = 0; r < 4; ++r) m[r] = ((uint64_t)m8.w[2 * r + 1] << 32) | m8.w[2 * r];
             walk_mask<kBatchDensity>(m, gt.start[wave][slot], i, dn);
             // Crowded buckets: their particles past the kSlots-th, a batch at a time.
+            // Mask mode: extraMask[i] bit k says extra particle k (enumerated bucket
+            // by bucket in visit order) is kept; all ones when there are more than
+            // kExtraBits extras.
+            uint32_t k = 0;
+            uint64_t bits = 0ull;
+            const bool anyCrowded = crowded != 0u;
             for (; crowded; crowded &= crowded - 1u) {
                 const uint32_t bi = (uint32_t)__builtin_ctz(crowded);
                 const uint32_t s0 = gt.start[wave][slot][bi], e = s0 + gt.cnt[wave][slot][bi];
@@ -637,14 +655,25 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : SPH_CW_DENSITY_WAVES) void k_d
                     uint32_t j[kBatchCrowded];
 #pragma unroll
                     for (uint32_t u = 0; u < kBatchCrowded; ++u) j[u] = jj + u < e ? jj + u : i;
-                    dn(j);  // padded entries (j = i) are never kept
+                    const uint32_t hits = dn(j);  // padded entries (j = i) are never kept
+                    if (!kList) {
+                        if (k < kExtraBits) bits |= (uint64_t)hits << k;
+                        k += min(kBatchCrowded, e - jj);
+                    }
                 }
             }
             flush();
+            if (!kList && anyCrowded) out->extraMask[i] = k > kExtraBits ? ~0ull : bits;
         });
     if (kStatic && ballot64(valid && (walls & ~kPoisonBit) != 0u) != 0ull && lane == 0u) out->heavy[blk] = 1u;
     if (!valid) return;
-    out->nbrCount[i] = (walls & kPoisonBit) ? kListPoison : nc;
+    if (kList) {
+        out->nbrCount[i] = (walls & kPoisonBit) ? kListPoison : nc;
+    } else {
+        uint4* mrow = reinterpret_cast<uint4*>(out->mask + (size_t)i * kCwMaskWords);
+        mrow[0] = make_uint4(m8.w[0], m8.w[1], m8.w[2], m8.w[3]);
+        mrow[1] = make_uint4(m8.w[4], m8.w[5], m8.w[6], m8.w[7]);
+    }
     if (kStatic) walk_walls<kBatchWall>(walls & ~kPoisonBit, cx, cy, cz, S, stable, dWalls);
     // A poisoned particle's density is NaN, as the pile it skipped makes it.
     const float rho = (walls & kPoisonBit) ? __builtin_nanf("")
@@ -660,10 +689,139 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : SPH_CW_DENSITY_WAVES) void k_d
     }
 }
 
-// Force pass of the cell-wave walk: each particle sums over the neighbour list
-// the density pass recorded (the same candidates in the same order as the
-// density pass kept them), so it needs no cell groups, tables or masks. A
-// particle whose list overflowed walks its 27 buckets by Morton runs instead.
+// Force pass of the cell-wave walk: the neighbour sums of one particle
+// (SF:557-589) and, in force_finish, the walls, the coincident-pair direction,
+// the force and the integration with the new key and histogram (SF:590-656).
+template <bool kMixed>
+struct ForceSums {
+    float4 xi, vi;   // the particle (pressure in xi.w)
+    uint32_t i;      // its index
+    float h, h2;
+    Acc<kMixed> fpx, fpy, fpz, fvx, fvy, fvz;
+    // Pressure weights of coincident pairs (r == 0): their direction is the
+    // particle's random direction (SF:566-570), the same for every such pair, so
+    // it is evaluated once after the walks.
+    float zeroSp = 0.f;
+    bool anyZero = false;
+
+    // One dynamic candidate j with its position (pressure in w) and velocity
+    // loaded. Candidates out of range (or j == i) add exact zeros: their
+    // pressure weight is selected away, since a NaN pressure times a zero weight
+    // would not be zero. A kept candidate contributes as in the reference, a zero
+    // kernel weight times its terms included.
+    __device__ void pair(uint32_t j, const float4 xj, const float4 vj) {
+        const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const bool ok = j != i && within_h2(r2, h2);
+        const float r = __builtin_amdgcn_sqrtf(r2);
+        const bool zero = r == 0.f;  // r2 == 0 implies r == 0
+        const float inv = zero ? 0.f : __builtin_amdgcn_rcpf(r);
+        const float q = h - r;
+        const bool in = inside_h(r, h);
+        const float w1 = in ? -3.0f * q * q : 0.f;
+        const float w2 = ok && in ? 6.0f * q : 0.f;
+        const float sp = ok ? (xi.w + xj.w) * w1 : 0.f;
+        fpx.add((dx * inv) * sp);
+        fpy.add((dy * inv) * sp);
+        fpz.add((dz * inv) * sp);
+        zeroSp += ok && zero ? sp : 0.f;
+        anyZero |= ok && zero;
+        fvx.add((vj.x - vi.x) * w2);
+        fvy.add((vj.y - vi.y) * w2);
+        fvz.add((vj.z - vi.z) * w2);
+    }
+    // A batch of candidates, their loads issued together.
+    template <class J>
+    __device__ void batch(const float4* __restrict__ pos, const float4* __restrict__ vel, const J& j) {
+        constexpr uint32_t B = sizeof(j) / sizeof(j[0]);
+        float4 xj[B], vj[B];
+#pragma unroll
+        for (uint32_t u = 0; u < B; ++u) {
+            xj[u] = ld16(pos, j[u]);
+            vj[u] = ld16(vel, j[u]);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < B; ++u) pair(j[u], xj[u], vj[u]);
+    }
+};
+
+// The rest of the force pass for particle f.i of scene `scene` (valid lanes
+// only; the active lanes of the wave are a prefix of it). `poisoned`: the
+// density pass skipped the NaN pile for it, so its acceleration is NaN, as the
+// pile makes it.
+template <bool kStatic, bool kDebug, bool kTri, bool kMixed>
+__device__ inline void force_finish(ForceSums<kMixed>& f, const SceneDev& S, uint32_t scene, uint32_t lane,
+                                    bool poisoned, const float4* __restrict__ wallSum,
+                                    const float4* __restrict__ wallZ, const uint32_t* __restrict__ ids,
+                                    const float* __restrict__ tris, float dt, float4* __restrict__ posT,
+                                    float4* __restrict__ velT, uint32_t* __restrict__ counts,
+                                    float4* __restrict__ accOut, uint32_t* __restrict__ nanNext) {
+    const uint32_t i = f.i;
+    const float4 xi = f.xi, vi = f.vi;
+    const float pi = xi.w;
+    const float rho = pi / S.k + S.rho0;  // SF:544
+    const float absP = fabsf(pi);
+    Acc<kMixed> fsx, fsy, fsz;
+    float sumD2s = 0.f;
+    if (kStatic) {
+        // Walls (SF:590-623): |p_i| times the density pass's geometric sums.
+        const float4 ws = ld16(wallSum, i);
+        fsx.add(absP * ws.x);
+        fsy.add(absP * ws.y);
+        fsz.add(absP * ws.z);
+        sumD2s = fabsf(ws.w);
+        if (__builtin_signbit(ws.w)) {
+            f.zeroSp += absP * wallZ[i].x;
+            f.anyZero = true;
+        }
+    }
+    if (f.anyZero) {
+        const float3 d = random_direction((float)(ids[i] - S.pBegin));
+        f.fpx.add(d.x * f.zeroSp);
+        f.fpy.add(d.y * f.zeroSp);
+        f.fpz.add(d.z * f.zeroSp);
+    }
+    // SF:626-638
+    const float sP = S.mass / (2.0f * rho) * S.C;
+    const float sV = S.mu * S.mass * S.C;
+    const float sVs = S.mu * S.mass * S.C * 0.0f;
+    float Fx = f.fpx.s * sP + f.fvx.s * sV + fsx.s * sP + (-vi.x * sumD2s) * sVs;
+    float Fy = f.fpy.s * sP + f.fvy.s * sV + fsy.s * sP + (-vi.y * sumD2s) * sVs;
+    float Fz = f.fpz.s * sP + f.fvz.s * sV + fsz.s * sP + (-vi.z * sumD2s) * sVs;
+    if (kTri) {
+        const float3 p3 = make_float3(xi.x, xi.y, xi.z);
+        for (uint32_t t = 0; t < S.triCount; ++t) {
+            const float3 fo = triangle_force(p3, tris + 9u * (S.triBegin + t));
+            Fx += fo.x;
+            Fy += fo.y;
+            Fz += fo.z;
+        }
+    }
+    const float nan = __builtin_nanf("");
+    const float ax = poisoned ? nan : Fx / rho + S.gx, ay = poisoned ? nan : Fy / rho + S.gy,
+                az = poisoned ? nan : Fz / rho + S.gz;
+    if (kDebug) accOut[i] = make_float4(ax, ay, az, 0.f);
+    // SF:641-645: symplectic Euler, then the key of the new cell.
+    const float h = S.h;
+    const float nvx = vi.x + ax * dt, nvy = vi.y + ay * dt, nvz = vi.z + az * dt;
+    const float nx = xi.x + nvx * dt, ny = xi.y + nvy * dt, nz = xi.z + nvz * dt;
+    const uint32_t ncode = morton3(cell_coord(nx, h), cell_coord(ny, h), cell_coord(nz, h));
+    const uint32_t key = fastmod(ncode, S.magicM, S.M);
+    posT[i] = make_float4(nx, ny, nz, pi);
+    velT[i] = make_float4(nvx, nvy, nvz, __uint_as_float(key));
+    const uint32_t gk = S.tBegin + key;
+    const Run run = wave_run(gk);
+    if (run.is_head) atomicAdd(&counts[gk], run.len);
+    // Next step's count of particles with a NaN coordinate in bucket 0. Lane 0
+    // is active: the active lanes are a prefix of the wave.
+    const uint64_t nb = ballot64(key == 0u && (nx != nx || ny != ny || nz != nz));
+    if (nb != 0ull && lane == 0u) atomicAdd(&nanNext[scene], (uint32_t)__popcll(nb));
+}
+
+// Force pass over the neighbour lists the density pass recorded (the same
+// candidates in the same order as the density pass kept them), so it needs no
+// cell groups, tables or masks. A particle whose list overflowed walks its 27
+// buckets by Morton runs instead.
 template <bool kStatic, bool kDebug, bool kTri, bool kMixed>
 __global__ __launch_bounds__(kBlock) void k_force_cw(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
                                                      const SceneDev* __restrict__ scenes,
@@ -684,56 +842,12 @@ __global__ __launch_bounds__(kBlock) void k_force_cw(const BlockDesc* __restrict
     const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
     const uint32_t i = bd.begin + threadIdx.x;
     if (i >= bd.end) return;  // the active lanes of a wave stay a prefix
-    const float4 xi = ld16(pos, i);
-    const float4 vi = ld16(vel, i);
-    const float pi = xi.w;  // this step's pressure (density pass)
-    const float h = S.h, h2 = S.h2;
-    const float rho = pi / S.k + S.rho0;  // SF:544
-    const float absP = fabsf(pi);
-    Acc<kMixed> fpx, fpy, fpz, fvx, fvy, fvz, fsx, fsy, fsz;
-    float sumD2s = 0.f;
-    // Pressure weights of coincident pairs (r == 0): their direction is the
-    // particle's random direction (SF:566-570), the same for every such pair, so
-    // it is evaluated once after the walks.
-    float zeroSp = 0.f;
-    bool anyZero = false;
-    // One dynamic candidate j (SF:557-589), with its position (pressure in w)
-    // and velocity already loaded. Candidates out of range (or j == i) add exact
-    // zeros: their pressure weight is selected away, since a NaN pressure times
-    // a zero weight would not be zero. A kept candidate contributes as in the
-    // reference, a zero kernel weight times its terms included.
-    auto pair_at = [&](uint32_t j, const float4 xj, const float4 vj) {
-        const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        const bool ok = j != i && within_h2(r2, h2);
-        const float r = __builtin_amdgcn_sqrtf(r2);
-        const bool zero = r == 0.f;  // r2 == 0 implies r == 0
-        const float inv = zero ? 0.f : __builtin_amdgcn_rcpf(r);
-        const float q = h - r;
-        const bool in = inside_h(r, h);
-        const float w1 = in ? -3.0f * q * q : 0.f;
-        const float w2 = ok && in ? 6.0f * q : 0.f;
-        const float sp = ok ? (pi + xj.w) * w1 : 0.f;
-        fpx.add((dx * inv) * sp);
-        fpy.add((dy * inv) * sp);
-        fpz.add((dz * inv) * sp);
-        zeroSp += ok && zero ? sp : 0.f;
-        anyZero |= ok && zero;
-        fvx.add((vj.x - vi.x) * w2);
-        fvy.add((vj.y - vi.y) * w2);
-        fvz.add((vj.z - vi.z) * w2);
-    };
-    auto pair4 = [&](const auto& j) {
-        constexpr uint32_t B = sizeof(j) / sizeof(j[0]);
-        float4 xj[B], vj[B];
-#pragma unroll
-        for (uint32_t u = 0; u < B; ++u) {
-            xj[u] = ld16(pos, j[u]);
-            vj[u] = ld16(vel, j[u]);
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < B; ++u) pair_at(j[u], xj[u], vj[u]);
-    };
+    ForceSums<kMixed> f;
+    f.xi = ld16(pos, i);
+    f.vi = ld16(vel, i);
+    f.i = i;
+    f.h = S.h;
+    f.h2 = S.h2;
     const uint32_t n = nbrCount[i];
     if (n <= kListCap) {
         // The list, kBatchForce entries at a time; a wave reads one 256-byte row
@@ -743,75 +857,98 @@ __global__ __launch_bounds__(kBlock) void k_force_cw(const BlockDesc* __restrict
             uint32_t j[kBatchForce];
 #pragma unroll
             for (uint32_t u = 0; u < kBatchForce; ++u) j[u] = s + u < n ? list[(s + u) << 6] : i;
-            pair4(j);
+            f.batch(pos, vel, j);
         }
     } else if (n != kListPoison) {
         // More neighbours than the list holds: every candidate of the 27 buckets
         // (SF:557-589), each bucket as often as the reference visits it.
-        const uint32_t lo = S.pBegin, hi = S.pBegin + S.pCount;
-        walk_runs(cell_coord(xi.x, h), cell_coord(xi.y, h), cell_coord(xi.z, h), S.M, S.magicM, tab + S.tBegin, lo,
-                  hi, [&](uint32_t b, uint32_t e) {
+        const float h = S.h;
+        walk_runs(cell_coord(f.xi.x, h), cell_coord(f.xi.y, h), cell_coord(f.xi.z, h), S.M, S.magicM,
+                  tab + S.tBegin, S.pBegin, S.pBegin + S.pCount, [&](uint32_t b, uint32_t e) {
                       for (uint32_t j = b; j < e; ++j) {
                           const uint32_t jj[1] = {j};
-                          pair4(jj);
+                          f.batch(pos, vel, jj);
                       }
                   });
     }
-    if (kStatic) {
-        // Walls (SF:590-623): |p_i| times the density pass's geometric sums.
-        const float4 ws = ld16(wallSum, i);
-        fsx.add(absP * ws.x);
-        fsy.add(absP * ws.y);
-        fsz.add(absP * ws.z);
-        sumD2s = fabsf(ws.w);
-        if (__builtin_signbit(ws.w)) {
-            zeroSp += absP * wallZ[i].x;
-            anyZero = true;
-        }
+    force_finish<kStatic, kDebug, kTri, kMixed>(f, S, bd.scene, lane, n == kListPoison, wallSum, wallZ, ids, tris,
+                                                dt, posT, velT, counts, accOut, nanNext);
+}
+
+// Force pass over the density pass's neighbour masks, for scenes too large for
+// the lists to stay in the caches: the wave rebuilds its cell groups and their
+// bucket tables (no tests), walks the same mask bits as the density pass, and
+// takes the crowded buckets' extra particles from the density pass's record.
+// Eight waves per SIMD (at most 64 VGPRs) in FP32; the mixed mode's Kahan sums
+// need more registers, so that mode settles for six.
+template <bool kStatic, bool kDebug, bool kTri, bool kMixed>
+__global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw_mask(
+    const BlockDesc* __restrict__ blocks, uint32_t numBlocks, const SceneDev* __restrict__ scenes,
+    const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ tab,
+    const float4* __restrict__ wallSum, const float4* __restrict__ wallZ, const uint32_t* __restrict__ mask,
+    const uint64_t* __restrict__ extraMask, const uint32_t* __restrict__ nanCnt, uint32_t* __restrict__ nanNext,
+    const uint32_t* __restrict__ ids, const float* __restrict__ tris, float dt, float4* __restrict__ posT,
+    float4* __restrict__ velT, uint32_t* __restrict__ counts, float4* __restrict__ accOut) {
+    __shared__ GroupTables gt;
+    const uint32_t blk = xcd_tile(blockIdx.x, numBlocks);
+    if (blk >= numBlocks) return;
+    const BlockDesc bd = blocks[blk];
+    const SceneDev& S = scenes[bd.scene];
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    const uint32_t wbegin = bd.begin + 64u * wave;
+    if (wbegin >= bd.end) return;
+    const uint32_t wcount = min(64u, bd.end - wbegin);
+    const uint32_t i = wbegin + lane;
+    const bool valid = lane < wcount;
+    const uint32_t iv = valid ? i : wbegin;
+    ForceSums<kMixed> f;
+    f.xi = ld16(pos, iv);
+    f.vi = ld16(vel, iv);
+    f.i = i;
+    f.h = S.h;
+    f.h2 = S.h2;
+    const uint32_t code = morton3(cell_coord(f.xi.x, f.h), cell_coord(f.xi.y, f.h), cell_coord(f.xi.z, f.h));
+    uint64_t m[4];
+    {
+        const uint4* mr = reinterpret_cast<const uint4*>(mask + (size_t)iv * kCwMaskWords);
+        const uint4 a = mr[0], b = mr[1];
+        m[0] = ((uint64_t)a.y << 32) | a.x;
+        m[1] = ((uint64_t)a.w << 32) | a.z;
+        m[2] = ((uint64_t)b.y << 32) | b.x;
+        m[3] = ((uint64_t)b.w << 32) | b.z;
     }
-    if (anyZero) {
-        const float3 d = random_direction((float)(ids[i] - S.pBegin));
-        fpx.add(d.x * zeroSp);
-        fpy.add(d.y * zeroSp);
-        fpz.add(d.z * zeroSp);
-    }
-    // SF:626-638
-    const float sP = S.mass / (2.0f * rho) * S.C;
-    const float sV = S.mu * S.mass * S.C;
-    const float sVs = S.mu * S.mass * S.C * 0.0f;
-    float Fx = fpx.s * sP + fvx.s * sV + fsx.s * sP + (-vi.x * sumD2s) * sVs;
-    float Fy = fpy.s * sP + fvy.s * sV + fsy.s * sP + (-vi.y * sumD2s) * sVs;
-    float Fz = fpz.s * sP + fvz.s * sV + fsz.s * sP + (-vi.z * sumD2s) * sVs;
-    if (kTri) {
-        const float3 p3 = make_float3(xi.x, xi.y, xi.z);
-        for (uint32_t t = 0; t < S.triCount; ++t) {
-            const float3 f = triangle_force(p3, tris + 9u * (S.triBegin + t));
-            Fx += f.x;
-            Fy += f.y;
-            Fz += f.z;
-        }
-    }
-    // A poisoned particle's acceleration is NaN, as the pile the density pass
-    // skipped makes it.
-    const float nan = __builtin_nanf("");
-    const bool poisoned = n == kListPoison;
-    const float ax = poisoned ? nan : Fx / rho + S.gx, ay = poisoned ? nan : Fy / rho + S.gy,
-                az = poisoned ? nan : Fz / rho + S.gz;
-    if (kDebug) accOut[i] = make_float4(ax, ay, az, 0.f);
-    // SF:641-645: symplectic Euler, then the key of the new cell.
-    const float nvx = vi.x + ax * dt, nvy = vi.y + ay * dt, nvz = vi.z + az * dt;
-    const float nx = xi.x + nvx * dt, ny = xi.y + nvy * dt, nz = xi.z + nvz * dt;
-    const uint32_t ncode = morton3(cell_coord(nx, h), cell_coord(ny, h), cell_coord(nz, h));
-    const uint32_t key = fastmod(ncode, S.magicM, S.M);
-    posT[i] = make_float4(nx, ny, nz, pi);
-    velT[i] = make_float4(nvx, nvy, nvz, __uint_as_float(key));
-    const uint32_t gk = S.tBegin + key;
-    const Run run = wave_run(gk);
-    if (run.is_head) atomicAdd(&counts[gk], run.len);
-    // Next step's count of particles with a NaN coordinate in bucket 0. Lane 0
-    // is active: the active lanes are a prefix of the wave.
-    const uint64_t nb = ballot64(key == 0u && (nx != nx || ny != ny || nz != nz));
-    if (nb != 0ull && lane == 0u) atomicAdd(&nanNext[bd.scene], (uint32_t)__popcll(nb));
+    auto pairs = [&](const auto& j) { f.batch(pos, vel, j); };
+    uint32_t noWalls = 0u;  // walls come from the density pass's sums; kPoisonBit
+    const uint64_t nanLanes = ballot64(valid && (f.xi.x != f.xi.x || f.xi.y != f.xi.y || f.xi.z != f.xi.z));
+    for_each_group<false>(
+        lane, wave, valid, code, lane_offsets(lane), S, tab + S.tBegin, nullptr, nanCnt[bd.scene], nanLanes,
+        S.pBegin, S.pBegin + S.pCount, gt, noWalls,
+        [&](uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, auto&& between) { between(); },
+        [&](uint32_t slot, uint32_t crowded) {
+            walk_mask<kBatchForceMask>(m, gt.start[wave][slot], i, pairs);
+            // Crowded buckets' extra particles: the density pass's neighbours
+            // among them when there are at most kExtraBits, else every one.
+            if (!crowded) return;
+            uint64_t xb = extraMask[i];
+            const bool all = xb == ~0ull;
+            for (; crowded; crowded &= crowded - 1u) {
+                const uint32_t bi = (uint32_t)__builtin_ctz(crowded);
+                const uint32_t s0 = gt.start[wave][slot][bi] + kSlots, ne = gt.cnt[wave][slot][bi] - kSlots;
+                for (uint32_t base = 0; base < ne; base += 32u) {
+                    const uint32_t c = min(32u, ne - base);
+                    const uint32_t low = c < 32u ? (1u << c) - 1u : ~0u;
+                    uint32_t w = all ? low : (uint32_t)xb & low;
+                    xb >>= c;
+                    for (; w; w &= w - 1u) {  // one at a time: fewer registers
+                        const uint32_t j[1] = {s0 + base + (uint32_t)__builtin_ctz(w)};
+                        f.batch(pos, vel, j);
+                    }
+                }
+            }
+        });
+    if (!valid) return;
+    force_finish<kStatic, kDebug, kTri, kMixed>(f, S, bd.scene, lane, (noWalls & kPoisonBit) != 0u, wallSum, wallZ,
+                                                ids, tris, dt, posT, velT, counts, accOut, nanNext);
 }
 
 }  // namespace sph

@@ -73,7 +73,9 @@ struct sph_engine {
     float* dens = nullptr;
     float4* acc = nullptr;
     uint32_t* mask = nullptr;       // candidate mask from the density pass (run, tile and cell-wave walks)
-    uint32_t* nbr = nullptr;        // cell-wave: neighbour lists (kListCap entries per particle)
+    bool cwList = false;            // cell-wave: neighbour lists (else masks) between the passes
+    uint64_t* extraMask = nullptr;  // cell-wave, mask mode: crowded buckets' neighbours
+    uint32_t* nbr = nullptr;        // cell-wave, list mode: neighbour lists (kListCap entries per particle)
     uint32_t* nbrCount = nullptr;   // cell-wave: list lengths
     DensityOut* densOut = nullptr;  // cell-wave density pass: its output pointers (device copy)
     uint32_t* nanCnt = nullptr;     // 2 x nScenes: NaN particles in bucket 0, per parity
@@ -175,6 +177,7 @@ void release(sph_engine* h) {
     dfree(h->acc);
     dfree(h->mask);
     dfree(h->nbr);
+    dfree(h->extraMask);
     dfree(h->nbrCount);
     dfree(h->densOut);
     dfree(h->nanCnt);
@@ -304,15 +307,19 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     const uint32_t* nanNow = h->nanCnt + (size_t)c * h->nScenes;
     uint32_t* nanNext = h->nanCnt + (size_t)(c ^ 1) * h->nScenes;
     if (walk != Walk::Cells) {
+#define SPH_DENS_CW(S, D, X, L)                                                                                \
+        hipLaunchKernelGGL((k_density_cw<S, D, X, L>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,   \
+                           h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->wallMap, h->tileOrder, h->nbr,  \
+                           nanNow, h->densOut)
 #define SPH_DENS(S, D)                                                                                         \
-    if (walk == Walk::CellWave && mixed)                                                                       \
-        hipLaunchKernelGGL((k_density_cw<S, D, true>), gridX, block, 0, h->stream, h->dBlocks,            \
-                           h->numBlocks, h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->wallMap,        \
-                           h->tileOrder, h->nbr, nanNow, h->densOut);                                      \
+    if (walk == Walk::CellWave && mixed && h->cwList)                                                          \
+        SPH_DENS_CW(S, D, true, true);                                                                         \
+    else if (walk == Walk::CellWave && mixed)                                                                  \
+        SPH_DENS_CW(S, D, true, false);                                                                        \
+    else if (walk == Walk::CellWave && h->cwList)                                                              \
+        SPH_DENS_CW(S, D, false, true);                                                                        \
     else if (walk == Walk::CellWave)                                                                           \
-        hipLaunchKernelGGL((k_density_cw<S, D, false>), gridX, block, 0, h->stream, h->dBlocks,            \
-                           h->numBlocks, h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->wallMap,        \
-                           h->tileOrder, h->nbr, nanNow, h->densOut);                                      \
+        SPH_DENS_CW(S, D, false, false);                                                                       \
     else if (walk == Walk::Tiles)                                                                              \
         hipLaunchKernelGGL((k_density_tile<S, D>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,       \
                            h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->mask, h->dens);                 \
@@ -333,6 +340,7 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
             }
         }
 #undef SPH_DENS
+#undef SPH_DENS_CW
     } else if (h->hasStatic) {
         if (dbg)
             hipLaunchKernelGGL((k_density<true, true>), grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->pos,
@@ -353,16 +361,24 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
 
     // Forces, integration, new keys, histogram into the other table.
     if ((rc = begin(SPH_STAGE_FORCE))) return rc;
+#define SPH_FORCE_LIST(S, D, T, X)                                                                                \
+        hipLaunchKernelGGL((k_force_cw<S, D, T, X>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,       \
+                           h->dScenes, h->pos, h->vel, h->tab[c], h->wallSum, h->wallZ, h->nbr, h->nbrCount,      \
+                           nanNext, h->ids[c], h->tris, dt, h->posT, h->velT, h->tab[c ^ 1], h->acc)
+#define SPH_FORCE_MASK(S, D, T, X)                                                                                \
+        hipLaunchKernelGGL((k_force_cw_mask<S, D, T, X>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,  \
+                           h->dScenes, h->pos, h->vel, h->tab[c], h->wallSum, h->wallZ, h->mask, h->extraMask,    \
+                           nanNow, nanNext, h->ids[c], h->tris, dt, h->posT, h->velT, h->tab[c ^ 1], h->acc)
 #define SPH_FORCE(S, D, T)                                                                                        \
     do {                                                                                                          \
-    if (walk == Walk::CellWave && mixed)                                                                          \
-        hipLaunchKernelGGL((k_force_cw<S, D, T, true>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,    \
-                           h->dScenes, h->pos, h->vel, h->tab[c], h->wallSum, h->wallZ, h->nbr, h->nbrCount,      \
-                           nanNext, h->ids[c], h->tris, dt, h->posT, h->velT, h->tab[c ^ 1], h->acc);             \
+    if (walk == Walk::CellWave && mixed && h->cwList)                                                             \
+        SPH_FORCE_LIST(S, D, T, true);                                                                            \
+    else if (walk == Walk::CellWave && mixed)                                                                     \
+        SPH_FORCE_MASK(S, D, T, true);                                                                            \
+    else if (walk == Walk::CellWave && h->cwList)                                                                 \
+        SPH_FORCE_LIST(S, D, T, false);                                                                           \
     else if (walk == Walk::CellWave)                                                                              \
-        hipLaunchKernelGGL((k_force_cw<S, D, T, false>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,   \
-                           h->dScenes, h->pos, h->vel, h->tab[c], h->wallSum, h->wallZ, h->nbr, h->nbrCount,      \
-                           nanNext, h->ids[c], h->tris, dt, h->posT, h->velT, h->tab[c ^ 1], h->acc);             \
+        SPH_FORCE_MASK(S, D, T, false);                                                                           \
     else if (walk == Walk::Tiles)                                                                                 \
         hipLaunchKernelGGL((k_force_tile<S, D, T>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,         \
                            h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->mask, h->ids[c], h->tris,  \
@@ -390,6 +406,8 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
         }
     }
 #undef SPH_FORCE
+#undef SPH_FORCE_LIST
+#undef SPH_FORCE_MASK
     SPH_HIP(h, hipGetLastError());
     if ((rc = end())) return rc;
 
@@ -675,8 +693,14 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
                   "every mask format uses 8 words per particle");
     if (!(flags & SPH_FLAG_CELL_WALK)) {
         SPH_ALLOC(h->mask, kMaskWords * std::max<uint64_t>(N, 1));
-        SPH_ALLOC(h->nbr, (uint64_t)std::max<uint32_t>(h->numBlocks, 1) * kWaves * kListCap * 64u);
-        SPH_ALLOC(h->nbrCount, std::max<uint64_t>(N, 1));
+        // Lists while they stay in the caches (4 B per neighbour), else masks (32 B per particle).
+        h->cwList = (flags & SPH_FLAG_CW_LISTS) || (!(flags & SPH_FLAG_CW_MASKS) && N <= SPH_CW_LIST_MAX);
+        if (h->cwList) {
+            SPH_ALLOC(h->nbr, (uint64_t)std::max<uint32_t>(h->numBlocks, 1) * kWaves * kListCap * 64u);
+            SPH_ALLOC(h->nbrCount, std::max<uint64_t>(N, 1));
+        } else {
+            SPH_ALLOC(h->extraMask, std::max<uint64_t>(N, 1));
+        }
     }
     if (flags & SPH_FLAG_RECORD_DEBUG) {
         SPH_ALLOC(h->dens, std::max<uint64_t>(N, 1));
@@ -708,8 +732,8 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
         SPH_HIP(h, hipMemcpy(h->tileOrder, identity.data(), sizeof(uint32_t) * identity.size(), hipMemcpyHostToDevice));
         SPH_HIP(h, hipMemset(h->heavy, 0, sizeof(uint32_t) * identity.size()));
     }
-    if (h->nbrCount) {
-        const DensityOut o{h->heavy, h->nbrCount, h->wallSum, h->wallZ, h->dens};
+    if (h->mask) {
+        const DensityOut o{h->heavy, h->nbrCount, h->mask, h->extraMask, h->wallSum, h->wallZ, h->dens};
         if ((rc = dalloc(h, &h->densOut, 1))) return rc;
         SPH_HIP(h, hipMemcpy(h->densOut, &o, sizeof(o), hipMemcpyHostToDevice));
     }

@@ -19,7 +19,8 @@ import pytest
 import oracle as orc
 from helpers import bucket_multisets_equal, check_buckets, expected_starts, max_norm_rel, max_rel
 from sphfluidsimulation_amd import SPHError, SimulationEngineHIP, scenes
-from sphfluidsimulation_amd._lib import (SPH_E_STATE, SPH_FLAG_CELL_WALK, SPH_FLAG_FP16_OFFSETS, SPH_FLAG_HIP_GRAPH,
+from sphfluidsimulation_amd._lib import (SPH_E_STATE, SPH_FLAG_CELL_WALK, SPH_FLAG_CW_MASKS, SPH_FLAG_FP16_OFFSETS,
+                                         SPH_FLAG_HIP_GRAPH,
                                          SPH_FLAG_RECORD_DEBUG, SPH_FLAG_RUN_WALK, SPH_FLAG_TILE_WALK,
                                          SPH_V_NONFINITE)
 
@@ -29,9 +30,13 @@ DENS_TOL = 1e-5
 ACC_TOL = 1e-5
 FP64_DENS_TOL = 1e-4
 
-# The three neighbour-pass variants: LDS-staged bucket tiles (default), Morton
-# runs from HBM, and the reference's cell-by-cell loop from HBM.
-WALKS = [pytest.param(0, id="cellwave"), pytest.param(SPH_FLAG_FP16_OFFSETS, id="mixed"),
+# The neighbour-pass variants: cell-group waves (the default; neighbour lists
+# between the passes at these sizes, masks with SPH_FLAG_CW_MASKS), the same in
+# mixed precision, LDS-staged bucket tiles, Morton runs from HBM, and the
+# reference's cell-by-cell loop from HBM.
+WALKS = [pytest.param(0, id="cellwave"), pytest.param(SPH_FLAG_CW_MASKS, id="cellwave-masks"),
+         pytest.param(SPH_FLAG_FP16_OFFSETS, id="mixed"),
+         pytest.param(SPH_FLAG_FP16_OFFSETS | SPH_FLAG_CW_MASKS, id="mixed-masks"),
          pytest.param(SPH_FLAG_TILE_WALK, id="tile"),
          pytest.param(SPH_FLAG_RUN_WALK, id="runs"),
          pytest.param(SPH_FLAG_CELL_WALK, id="cells")]
