@@ -1,4 +1,4 @@
-// LDS-staged neighbour passes ("tile walk", the default).
+// LDS-staged neighbour passes ("tile walk", selected with SPH_FLAG_TILE_WALK).
 //
 // A workgroup owns 256 consecutive particles in bucket order. Before any
 // neighbour is read it builds, in LDS, the set of distinct buckets its
