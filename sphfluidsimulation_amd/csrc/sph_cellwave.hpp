@@ -87,6 +87,7 @@ __device__ inline uint64_t pick4(const uint64_t (&m)[4], uint32_t r) {
 // the compiler from sinking a batch of loads into the predicated branches that
 // use them (one wait for four loads would become four waits).
 __device__ inline void settle(uint32_t (&j)[2]) { asm volatile("" : "+v"(j[0]), "+v"(j[1])); }
+__device__ inline void settle(uint32_t (&j)[3]) { asm volatile("" : "+v"(j[0]), "+v"(j[1]), "+v"(j[2])); }
 __device__ inline void settle(uint32_t (&j)[4]) {
     asm volatile("" : "+v"(j[0]), "+v"(j[1]), "+v"(j[2]), "+v"(j[3]));
 }
@@ -133,30 +134,46 @@ constexpr uint32_t kListPoison = 0xFFFFFFFFu;
 
 // Walks the set bits of a neighbour mask B at a time: f(j) gets B candidate
 // indices (bucket start of the bit's bucket + slot), `self` standing in for
-// missing ones, so the B loads of a call are issued together. Rounds are walked
-// in order and bits in ascending order within a round.
-template <uint32_t B, class F>
+// missing ones, so the B loads of a call are issued together. Each lane walks
+// its rounds in order and the bits of a round in ascending order, but moves on
+// to its next nonzero round as soon as the current one is spent, whatever the
+// other lanes do: the wave iterates as often as its busiest lane needs, not
+// once per round for the busiest lane of each round. With kFlat a batch also
+// crosses from one round into the next (fewer iterations, more work to pick
+// each bit); without it a batch stays in one round.
+template <uint32_t B, bool kFlat, class F>
 __device__ inline void walk_mask(const uint64_t (&m)[4], const uint32_t* start, uint32_t self, F&& f) {
-#pragma unroll 1
-    for (uint32_t r = 0; r < 4; ++r) {
-        uint64_t mm = pick4(m, r);
-        const uint32_t* st = start + 8u * r;
-        while (mm) {
-            uint32_t l[B], b[B], j[B];
-            bool ok[B];
-#pragma unroll
-            for (uint32_t u = 0; u < B; ++u) {
-                ok[u] = mm != 0ull;
-                l[u] = ok[u] ? (uint32_t)__builtin_ctzll(mm) : 0u;
-                mm &= mm - 1ull;
-            }
-#pragma unroll
-            for (uint32_t u = 0; u < B; ++u) b[u] = st[l[u] >> 3];  // valid slots: read unconditionally
-            settle(b);
-#pragma unroll
-            for (uint32_t u = 0; u < B; ++u) j[u] = ok[u] ? b[u] + (l[u] & 7u) : self;
-            f(j);
+    // Bit r: round r still to walk (its word is nonzero).
+    uint32_t left = (m[0] != 0ull ? 1u : 0u) | (m[1] != 0ull ? 2u : 0u) | (m[2] != 0ull ? 4u : 0u) |
+                    (m[3] != 0ull ? 8u : 0u);
+    uint64_t mm = 0ull;  // the rest of the round being walked
+    uint32_t base = 0u;  // its first bucket (8 per round)
+    auto refill = [&]() {
+        if (mm == 0ull && left != 0u) {
+            const uint32_t r = (uint32_t)__builtin_ctz(left);
+            left &= left - 1u;
+            mm = pick4(m, r);
+            base = 8u * r;
         }
+    };
+    while (true) {
+        refill();
+        if (mm == 0ull) break;
+        uint32_t l[B], b[B], j[B];
+        bool ok[B];
+#pragma unroll
+        for (uint32_t u = 0; u < B; ++u) {
+            if (kFlat && u > 0) refill();
+            ok[u] = mm != 0ull;
+            l[u] = ok[u] ? base * 8u + (uint32_t)__builtin_ctzll(mm) : 0u;
+            mm &= mm - 1ull;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < B; ++u) b[u] = start[l[u] >> 3];  // valid slots: read unconditionally
+        settle(b);
+#pragma unroll
+        for (uint32_t u = 0; u < B; ++u) j[u] = ok[u] ? b[u] + (l[u] & 7u) : self;
+        f(j);
     }
 }
 
@@ -640,7 +657,7 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : SPH_CW_DENSITY_WAVES) void k_d
             uint64_t m[4];
 #pragma unroll
             for (uint32_t r = 0; r < 4; ++r) m[r] = ((uint64_t)m8.w[2 * r + 1] << 32) | m8.w[2 * r];
-            walk_mask<kBatchDensity>(m, gt.start[wave][slot], i, dn);
+            walk_mask<kBatchDensity, false>(m, gt.start[wave][slot], i, dn);
             // Crowded buckets: their particles past the kSlots-th, a batch at a time.
             // Mask mode: extraMask[i] bit k says extra particle k (enumerated bucket
             // by bucket in visit order) is kept; all ones when there are more than
@@ -925,7 +942,7 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw_mask(
         S.pBegin, S.pBegin + S.pCount, gt, noWalls,
         [&](uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, auto&& between) { between(); },
         [&](uint32_t slot, uint32_t crowded) {
-            walk_mask<kBatchForceMask>(m, gt.start[wave][slot], i, pairs);
+            walk_mask<kBatchForceMask, true>(m, gt.start[wave][slot], i, pairs);
             // Crowded buckets' extra particles: the density pass's neighbours
             // among them when there are at most kExtraBits, else every one.
             if (!crowded) return;
