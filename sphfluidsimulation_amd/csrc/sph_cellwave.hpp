@@ -98,7 +98,7 @@ __device__ inline void settle(uint32_t (&j)[8]) {
 // Candidates per batch of the mask walk: the density pass loads one float4 per
 // candidate, the force pass two, so the force pass holds fewer in registers.
 #ifndef SPH_CW_BATCH_DENSITY
-#define SPH_CW_BATCH_DENSITY 4
+#define SPH_CW_BATCH_DENSITY 3
 #endif
 #ifndef SPH_CW_BATCH_FORCE
 #define SPH_CW_BATCH_FORCE 4
@@ -112,7 +112,10 @@ constexpr uint32_t kBatchCrowded = SPH_CW_BATCH_CROWDED;
 constexpr uint32_t kBatchForce = SPH_CW_BATCH_FORCE;
 // Force pass over masks: it also holds the cell groups and the masks in
 // registers, so it takes two candidates at a time.
-constexpr uint32_t kBatchForceMask = 2;
+#ifndef SPH_CW_BATCH_FORCE_MASK
+#define SPH_CW_BATCH_FORCE_MASK 2
+#endif
+constexpr uint32_t kBatchForceMask = SPH_CW_BATCH_FORCE_MASK;
 // Crowded buckets, mask mode: the density pass records which of a particle's
 // first kExtraBits extra candidates are neighbours, for the force pass.
 constexpr uint32_t kExtraBits = 64;
