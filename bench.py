@@ -197,8 +197,11 @@ def main():
     # Timed region: events around the stages we report (the engine's own stream).
     # hipGraph replay carries no per-kernel events, so with --graph the timed
     # region replays graphs untouched and the stage times come from an eager
-    # window of the same length (at most 100 steps) right after it.
+    # window of the same length (at most 100 steps) right after it. With one
+    # engine per scene the events would add host work per engine to the timed
+    # region, so those runs take their stage times from such a window too.
     prof_stages = True if args.profile_all else ["force", "density"]
+    events_after = args.graph or plain
 
     def collect():
         out = {}
@@ -210,7 +213,7 @@ def main():
             e.set_profiling(None)
         return out
 
-    if not args.graph:
+    if not events_after:
         for e in engines:
             e.set_profiling(prof_stages)
     t0 = time.perf_counter()
@@ -218,7 +221,7 @@ def main():
     for e in engines:
         e.synchronize()
     elapsed = time.perf_counter() - t0
-    if args.graph:
+    if events_after:
         for e in engines:
             e.set_profiling(prof_stages)
         run(min(steps, 100))
@@ -262,7 +265,7 @@ def main():
             "launch": "one engine per scene" if plain else "batched" if len(scene_list) > 1 else "single scene",
             "dt": dt,
             "hip_graph": bool(args.graph),
-            "stage_times_from": "eager window after the timed region" if args.graph else "timed region",
+            "stage_times_from": "eager window after the timed region" if events_after else "timed region",
             "walk": args.walk,
             "autotune": bool(args.autotune),
             "precision": "mixed (fp16 filter, kahan sums)" if args.mixed else "fp32",
