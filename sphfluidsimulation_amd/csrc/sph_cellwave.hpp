@@ -31,7 +31,17 @@ namespace sph {
 constexpr uint32_t kWaves = kBlock / 64;  // waves per workgroup
 constexpr uint32_t kGroupBatch = 16;      // groups whose bucket tables a wave keeps at once
 constexpr uint32_t kCwMaskWords = 8;      // 4 rounds x 64 bits per particle
-constexpr uint32_t kSlots = 8;            // candidates per bucket in the round layout
+// Candidates per bucket in the round layout: position l = 64 r + lane of round r
+// holds slot l % kSlots of bucket l / kSlots. Nine slots (243 positions) still
+// fit four rounds and take the ninth particle of a compressed cell without the
+// per-lane walk of crowded buckets.
+#ifndef SPH_CW_SLOTS
+#define SPH_CW_SLOTS 9
+#endif
+constexpr uint32_t kSlots = SPH_CW_SLOTS;
+static_assert(kSlots == 8 || kSlots == 9, "27 buckets of kSlots positions must fit four rounds of 64");
+// Bucket of layout position l < 27 kSlots (l / 9 as (57 l) >> 9, exact below 243).
+__device__ __host__ inline uint32_t slot_bucket(uint32_t l) { return kSlots == 8 ? l >> 3 : (l * 57u) >> 9; }
 
 // Per-wave scratch: bucket start and count of the 27 neighbour buckets of up to
 // kGroupBatch groups.
@@ -53,21 +63,39 @@ struct Mask8 {
 // with the lane select in m0, since the data operand already uses the one scalar
 // operand a VALU instruction may read; the s_nop covers the wait states between
 // writing m0 and reading it as a lane select).
-// Round 3 holds buckets 24..26 on lanes 0..23 only, so the high word of its
-// ballot is always zero and w[7] is never written (it stays zero).
+// With 8 slots, round 3 holds buckets 24..26 on lanes 0..23 only, so the high
+// word of its ballot is always zero and w[7] is never written (it stays zero).
 __device__ inline void writelane4(Mask8& m, const uint64_t (&b)[4], uint32_t t) {
-    asm volatile(
-        "s_nop 4\n\t"
-        "v_writelane_b32 %0, %7, m0\n\t"
-        "v_writelane_b32 %1, %8, m0\n\t"
-        "v_writelane_b32 %2, %9, m0\n\t"
-        "v_writelane_b32 %3, %10, m0\n\t"
-        "v_writelane_b32 %4, %11, m0\n\t"
-        "v_writelane_b32 %5, %12, m0\n\t"
-        "v_writelane_b32 %6, %13, m0"
-        : "+v"(m.w[0]), "+v"(m.w[1]), "+v"(m.w[2]), "+v"(m.w[3]), "+v"(m.w[4]), "+v"(m.w[5]), "+v"(m.w[6])
-        : "s"((uint32_t)b[0]), "s"((uint32_t)(b[0] >> 32)), "s"((uint32_t)b[1]), "s"((uint32_t)(b[1] >> 32)),
-          "s"((uint32_t)b[2]), "s"((uint32_t)(b[2] >> 32)), "s"((uint32_t)b[3]), "{m0}"(t));
+    if constexpr (kSlots == 8) {
+        asm volatile(
+            "s_nop 4\n\t"
+            "v_writelane_b32 %0, %7, m0\n\t"
+            "v_writelane_b32 %1, %8, m0\n\t"
+            "v_writelane_b32 %2, %9, m0\n\t"
+            "v_writelane_b32 %3, %10, m0\n\t"
+            "v_writelane_b32 %4, %11, m0\n\t"
+            "v_writelane_b32 %5, %12, m0\n\t"
+            "v_writelane_b32 %6, %13, m0"
+            : "+v"(m.w[0]), "+v"(m.w[1]), "+v"(m.w[2]), "+v"(m.w[3]), "+v"(m.w[4]), "+v"(m.w[5]), "+v"(m.w[6])
+            : "s"((uint32_t)b[0]), "s"((uint32_t)(b[0] >> 32)), "s"((uint32_t)b[1]), "s"((uint32_t)(b[1] >> 32)),
+              "s"((uint32_t)b[2]), "s"((uint32_t)(b[2] >> 32)), "s"((uint32_t)b[3]), "{m0}"(t));
+    } else {
+        asm volatile(
+            "s_nop 4\n\t"
+            "v_writelane_b32 %0, %8, m0\n\t"
+            "v_writelane_b32 %1, %9, m0\n\t"
+            "v_writelane_b32 %2, %10, m0\n\t"
+            "v_writelane_b32 %3, %11, m0\n\t"
+            "v_writelane_b32 %4, %12, m0\n\t"
+            "v_writelane_b32 %5, %13, m0\n\t"
+            "v_writelane_b32 %6, %14, m0\n\t"
+            "v_writelane_b32 %7, %15, m0"
+            : "+v"(m.w[0]), "+v"(m.w[1]), "+v"(m.w[2]), "+v"(m.w[3]), "+v"(m.w[4]), "+v"(m.w[5]), "+v"(m.w[6]),
+              "+v"(m.w[7])
+            : "s"((uint32_t)b[0]), "s"((uint32_t)(b[0] >> 32)), "s"((uint32_t)b[1]), "s"((uint32_t)(b[1] >> 32)),
+              "s"((uint32_t)b[2]), "s"((uint32_t)(b[2] >> 32)), "s"((uint32_t)b[3]), "s"((uint32_t)(b[3] >> 32)),
+              "{m0}"(t));
+    }
 }
 
 // Record j of an array of 16-byte records through a 32-bit byte offset (the
@@ -150,32 +178,35 @@ __device__ inline void walk_mask(const uint64_t (&m)[4], const uint32_t* start, 
     uint32_t left = (m[0] != 0ull ? 1u : 0u) | (m[1] != 0ull ? 2u : 0u) | (m[2] != 0ull ? 4u : 0u) |
                     (m[3] != 0ull ? 8u : 0u);
     uint64_t mm = 0ull;  // the rest of the round being walked
-    uint32_t base = 0u;  // its first bucket (8 per round)
+    uint32_t base = 0u;  // its first layout position
     auto refill = [&]() {
         if (mm == 0ull && left != 0u) {
             const uint32_t r = (uint32_t)__builtin_ctz(left);
             left &= left - 1u;
             mm = pick4(m, r);
-            base = 8u * r;
+            base = 64u * r;
         }
     };
     while (true) {
         refill();
         if (mm == 0ull) break;
-        uint32_t l[B], b[B], j[B];
+        uint32_t l[B], b[B], c[B], j[B];
         bool ok[B];
 #pragma unroll
         for (uint32_t u = 0; u < B; ++u) {
             if (kFlat && u > 0) refill();
             ok[u] = mm != 0ull;
-            l[u] = ok[u] ? base * 8u + (uint32_t)__builtin_ctzll(mm) : 0u;
+            l[u] = ok[u] ? base + (uint32_t)__builtin_ctzll(mm) : 0u;
             mm &= mm - 1ull;
         }
 #pragma unroll
-        for (uint32_t u = 0; u < B; ++u) b[u] = start[l[u] >> 3];  // valid slots: read unconditionally
+        for (uint32_t u = 0; u < B; ++u) {
+            c[u] = slot_bucket(l[u]);
+            b[u] = start[c[u]];  // valid slots: read unconditionally
+        }
         settle(b);
 #pragma unroll
-        for (uint32_t u = 0; u < B; ++u) j[u] = ok[u] ? b[u] + (l[u] & 7u) : self;
+        for (uint32_t u = 0; u < B; ++u) j[u] = ok[u] ? b[u] + (l[u] - kSlots * c[u]) : self;
         f(j);
     }
 }
@@ -219,20 +250,20 @@ __device__ inline half2v h2_from(uint32_t b) { return __builtin_bit_cast(half2v,
 // it, as they should; the exact FP32 test follows in the mask walk.
 constexpr float kFilterR2 = 1.02f;
 
-// Candidate of round r on this lane: bucket 8r + lane/8, slot lane%8.
+// Candidate of round r on this lane: layout position 64 r + lane.
 struct RoundCand {
     uint32_t j;
     bool valid;
 };
 
 __device__ inline RoundCand round_candidate(uint32_t r, uint32_t lane, uint32_t start27, uint32_t cnt27) {
-    const uint32_t bi = 8u * r + (lane >> 3);
+    const uint32_t l = 64u * r + lane, bi = slot_bucket(l), k = l - kSlots * bi;
     const int src = (int)(bi < 27u ? bi : 0u);
     const uint32_t bs = (uint32_t)__shfl((int)start27, src);
     const uint32_t bc = (uint32_t)__shfl((int)cnt27, src);
     RoundCand c;
-    c.valid = bi < 27u && (lane & 7u) < bc;
-    c.j = bs + (lane & 7u);
+    c.valid = bi < 27u && k < bc;
+    c.j = bs + k;
     return c;
 }
 
