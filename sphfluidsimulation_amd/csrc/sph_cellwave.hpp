@@ -5,17 +5,19 @@
 // share the same 27 neighbour buckets. For each group the wave:
 //   1. computes the 27 neighbour buckets on lanes 0..26, in the reference's
 //      x-y-z visit order (SF:397-404), with their particle ranges;
-//   2. lays the candidates out as four rounds of 64 lanes, round r holding
-//      buckets 8r .. 8r+7 with up to 8 particles each (lane = 8 * bucket + slot),
-//      so every lane loads one candidate and the loads are contiguous per bucket;
+//   2. lays the candidates out as four rounds of 64 lanes, up to kSlots (9)
+//      particles of each bucket in consecutive positions (position 64 r + lane
+//      of round r is slot l % kSlots of bucket l / kSlots), so every lane loads
+//      one candidate and the loads are contiguous per bucket;
 //   3. tests every candidate against every target of the group, the target's
 //      position being wave-uniform: a handful of VALU operations per pair and
 //      no per-lane loop bookkeeping. Each (target, round) test is a 64-bit
 //      ballot, written into the target's lane.
 // After the groups, every lane holds its own particle's neighbour mask (4 x 64
 // bits over the candidate layout) and walks only the set bits. The density pass
-// stores the mask; the force pass reads it instead of testing again. Buckets
-// with more than 8 particles (crowded cells) leave their extra particles out of
+// stores the mask or a list of the kept candidates; the force pass reads it
+// instead of testing again. Buckets with more than kSlots particles (crowded
+// cells, buckets shared by two cells) leave their extra particles out of
 // the layout; each lane of such a group tests them directly, so every bucket is
 // visited exactly as often as in the reference, colliding buckets included.
 // Static (wall) neighbours: each group reads the wall-cell bitmap bits of its 27
