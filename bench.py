@@ -122,6 +122,9 @@ def main():
     ap.add_argument("--walk", default="cellwave", choices=["cellwave", "tile", "runs", "cells"],
                     help="neighbour passes: cell-group waves (default), LDS-staged bucket tiles, "
                          "HBM Morton runs, or cell by cell")
+    ap.add_argument("--neighbours", choices=["auto", "lists", "masks"], default="auto",
+                    help="cell-wave walk: how the density pass hands neighbours to the force pass "
+                         "(auto: lists up to 1.5M particles, masks above)")
     ap.add_argument("--mixed", action="store_true",
                     help="mixed precision: FP16-offset candidate filter, Kahan-compensated neighbour sums")
     ap.add_argument("--autotune", action="store_true",
@@ -140,8 +143,9 @@ def main():
 
 
     from sphfluidsimulation_amd import SimulationEngineHIP, scenes
-    from sphfluidsimulation_amd._lib import (SPH_FLAG_CELL_WALK, SPH_FLAG_FP16_OFFSETS, SPH_FLAG_HIP_GRAPH,
-                                             SPH_FLAG_RUN_WALK, SPH_FLAG_TILE_WALK)
+    from sphfluidsimulation_amd._lib import (SPH_FLAG_CELL_WALK, SPH_FLAG_CW_LISTS, SPH_FLAG_CW_MASKS,
+                                             SPH_FLAG_FP16_OFFSETS, SPH_FLAG_HIP_GRAPH, SPH_FLAG_RUN_WALK,
+                                             SPH_FLAG_TILE_WALK)
 
     if args.copy_probe:
         e = SimulationEngineHIP(0)
@@ -162,7 +166,9 @@ def main():
 
     flags = ((SPH_FLAG_HIP_GRAPH if args.graph else 0) | (SPH_FLAG_CELL_WALK if args.walk == "cells" else 0)
              | (SPH_FLAG_RUN_WALK if args.walk == "runs" else 0) | (SPH_FLAG_TILE_WALK if args.walk == "tile" else 0)
-             | (SPH_FLAG_FP16_OFFSETS if args.mixed else 0))
+             | (SPH_FLAG_FP16_OFFSETS if args.mixed else 0)
+             | (SPH_FLAG_CW_LISTS if args.neighbours == "lists" else 0)
+             | (SPH_FLAG_CW_MASKS if args.neighbours == "masks" else 0))
     plain = args.config.endswith("-plain")
     engines = []
     if plain:
@@ -267,6 +273,7 @@ def main():
             "hip_graph": bool(args.graph),
             "stage_times_from": "eager window after the timed region" if events_after else "timed region",
             "walk": args.walk,
+            "neighbours": args.neighbours,
             "autotune": bool(args.autotune),
             "precision": "mixed (fp16 filter, kahan sums)" if args.mixed else "fp32",
         },
