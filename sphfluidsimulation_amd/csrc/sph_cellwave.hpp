@@ -446,12 +446,12 @@ __device__ inline void walk_walls(uint32_t walls, uint32_t cx, uint32_t cy, uint
 // blocks come first, in block order, then the rest. One workgroup per chunk;
 // clears the marks. Any permutation within the chunks gives the same results;
 // this one only changes when each block runs.
-__global__ __launch_bounds__(256) void k_tile_order(uint32_t* __restrict__ heavy, uint32_t numBlocks,
-                                                    uint32_t* __restrict__ order) {
+__device__ inline void tile_order_chunk(uint32_t k, uint32_t* __restrict__ heavy, uint32_t numBlocks,
+                                        uint32_t* __restrict__ order) {
     __shared__ uint32_t sWave[4];
     __shared__ uint32_t sTotal;
     const uint32_t chunk = (numBlocks + 7u) >> 3;
-    const uint32_t base = blockIdx.x * chunk;
+    const uint32_t base = k * chunk;
     const uint32_t valid = base < numBlocks ? min(chunk, numBlocks - base) : 0u;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     // Pass 1: how many marked blocks the chunk holds.
@@ -487,6 +487,24 @@ __global__ __launch_bounds__(256) void k_tile_order(uint32_t* __restrict__ heavy
         }
         markedBefore += tileMarks;
     }
+}
+
+// The scatter (workgroups 0..numBlocks-1) and, on eight more workgroups, the
+// next step's launch order (tile_order_chunk k on workgroup numBlocks + k), in
+// one launch: both only need this step's density and force passes done.
+static_assert(kBlock == 256, "the launch order's workgroups take the scatter's size");
+__global__ __launch_bounds__(kBlock) void k_scatter_order(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
+                                                          const SceneDev* __restrict__ scenes,
+                                                          const float4* __restrict__ posT,
+                                                          const float4* __restrict__ velT,
+                                                          const uint32_t* __restrict__ idIn,
+                                                          uint32_t* __restrict__ cursor, float4* __restrict__ pos,
+                                                          float4* __restrict__ vel, uint32_t* __restrict__ idOut,
+                                                          uint32_t* __restrict__ heavy, uint32_t* __restrict__ order) {
+    if (blockIdx.x < numBlocks)
+        scatter_block(blockIdx.x, blocks, scenes, posT, velT, idIn, cursor, pos, vel, idOut);
+    else
+        tile_order_chunk(blockIdx.x - numBlocks, heavy, numBlocks, order);
 }
 
 // Where the density pass writes its per-particle results (read once, at the

@@ -63,7 +63,7 @@ struct sph_engine {
     float4* spos = nullptr;
     uint32_t* stab = nullptr;
     uint32_t* wallMap = nullptr;  // per scene: bit wall_hash(Morton code of c) set for every cell c holding walls
-    uint32_t* tileOrder = nullptr;  // cell-wave launch order (k_tile_order), gridX entries
+    uint32_t* tileOrder = nullptr;  // cell-wave launch order (tile_order_chunk), numBlocks entries
     uint32_t* heavy = nullptr;      // per block: particles near walls in the last density pass
     float4* wallSum = nullptr;      // cell-wave: the walls' geometric sums of each particle, for the force pass
     float4* wallZ = nullptr;        // and the coincident-pair weight where one occurred
@@ -421,14 +421,17 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
 
     // Move the state into bucket order.
     if ((rc = begin(SPH_STAGE_SCATTER))) return rc;
-    hipLaunchKernelGGL(k_scatter, grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->posT, h->velT, h->ids[c],
-                       h->tab[c ^ 1], h->pos, h->vel, h->ids[c ^ 1]);
-    SPH_HIP(h, hipGetLastError());
-    // Next step's cell-wave launch order: workgroups near walls first.
+    // With the cell-wave walk and walls, the same launch also computes the next
+    // step's launch order (workgroups near walls first) on eight more workgroups.
     if (walk == Walk::CellWave && h->hasStatic) {
-        hipLaunchKernelGGL(k_tile_order, dim3(8), dim3(256), 0, h->stream, h->heavy, h->numBlocks, h->tileOrder);
-        SPH_HIP(h, hipGetLastError());
+        hipLaunchKernelGGL(k_scatter_order, dim3(h->numBlocks + 8u), block, 0, h->stream, h->dBlocks, h->numBlocks,
+                           h->dScenes, h->posT, h->velT, h->ids[c], h->tab[c ^ 1], h->pos, h->vel, h->ids[c ^ 1],
+                           h->heavy, h->tileOrder);
+    } else {
+        hipLaunchKernelGGL(k_scatter, grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->posT, h->velT,
+                           h->ids[c], h->tab[c ^ 1], h->pos, h->vel, h->ids[c ^ 1]);
     }
+    SPH_HIP(h, hipGetLastError());
     if ((rc = end())) return rc;
     return SPH_OK;
 }

@@ -758,13 +758,12 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* __restrict__ da
 // atomic and keeps its wave order, so afterwards cursor[b] is the bucket start
 // (the reference's final hashMap). The particle state itself moves, so the next
 // step's neighbour reads are contiguous per bucket.
-__global__ __launch_bounds__(kBlock) void k_scatter(const BlockDesc* __restrict__ blocks,
-                                                    const SceneDev* __restrict__ scenes,
-                                                    const float4* __restrict__ posT, const float4* __restrict__ velT,
-                                                    const uint32_t* __restrict__ idIn, uint32_t* __restrict__ cursor,
-                                                    float4* __restrict__ pos, float4* __restrict__ vel,
-                                                    uint32_t* __restrict__ idOut) {
-    const BlockDesc bd = blocks[blockIdx.x];
+__device__ inline void scatter_block(uint32_t blk, const BlockDesc* __restrict__ blocks,
+                                     const SceneDev* __restrict__ scenes, const float4* __restrict__ posT,
+                                     const float4* __restrict__ velT, const uint32_t* __restrict__ idIn,
+                                     uint32_t* __restrict__ cursor, float4* __restrict__ pos,
+                                     float4* __restrict__ vel, uint32_t* __restrict__ idOut) {
+    const BlockDesc bd = blocks[blk];
     const SceneDev& S = scenes[bd.scene];
     const uint32_t tBegin = S.tBegin;
     const uint32_t i = bd.begin + threadIdx.x;
@@ -785,6 +784,15 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const BlockDesc* __restrict_
     pos[dst] = p;
     vel[dst] = v;
     idOut[dst] = id;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter(const BlockDesc* __restrict__ blocks,
+                                                    const SceneDev* __restrict__ scenes,
+                                                    const float4* __restrict__ posT, const float4* __restrict__ velT,
+                                                    const uint32_t* __restrict__ idIn, uint32_t* __restrict__ cursor,
+                                                    float4* __restrict__ pos, float4* __restrict__ vel,
+                                                    uint32_t* __restrict__ idOut) {
+    scatter_block(blockIdx.x, blocks, scenes, posT, velT, idIn, cursor, pos, vel, idOut);
 }
 
 // ---------------------------------------------------------------------------
