@@ -326,9 +326,10 @@ def _occupancy_scene(kind):
     # aliased: two blocks whose cells share Morton codes, so cell groups hold
     # particles of cells 1024 apart;
     # crowded: 27 particles per cell on average, so every bucket holds more than
-    # the 8 candidates per bucket the cell-group layout keeps in registers;
-    # dense: about 11 per cell;
-    # sparse: about 2 per cell, so a wave of 64 particles spans more than the 32
+    # the 9 candidates per bucket the cell-group layout keeps in registers;
+    # dense: about 11 per cell (buckets of 9, 10 and more, either side of the
+    # layout's limit);
+    # sparse: about 2 per cell, so a wave of 64 particles spans more than the 16
     # cell groups whose bucket tables it keeps at once.
     if kind == "aliased":
         # Two copies of a block 1024 cells apart in x: equal Morton codes, so both
@@ -343,7 +344,7 @@ def _occupancy_scene(kind):
         return scenes.dam_break(24, 24, 24, (2.4, 2.4, 2.4), spacing=0.1 / 3.0, jitter_frac=0.3, seed=11,
                                 shuffle=True, name="crowded-cells")
     if kind == "dense":
-        # About 11 per cell: a particle's extra candidates past the 8 per bucket
+        # About 11 per cell: a particle's extra candidates past the 9 per bucket
         # number from a few to past 32, so both of the force pass's ways of
         # taking them (the density pass's record, or every one) run.
         return scenes.dam_break(26, 26, 26, (2.4, 2.4, 2.4), spacing=0.1 / 2.2, jitter_frac=0.3, seed=14,
