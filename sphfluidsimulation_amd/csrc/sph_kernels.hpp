@@ -633,6 +633,20 @@ __device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
     return v;
 }
 
+// Inclusive scan across the wave: DPP row shifts within each row of 16 lanes,
+// then the rows' totals (lanes 15, 31, 47) carried into the later rows.
+__device__ inline uint32_t wave_inclusive_scan_dpp(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);  // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);  // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);  // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);  // row_shr:8
+    const uint32_t lane = __lane_id();
+    const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 15);
+    const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 31);
+    const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 47);
+    return v + (lane >= 16u ? r0 : 0u) + (lane >= 32u ? r1 : 0u) + (lane >= 48u ? r2 : 0u);
+}
+
 __device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t& total, uint32_t* s_w) {
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t inc = wave_inclusive_scan(x);
@@ -651,7 +665,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* __restrict__ da
                                                        uint32_t* __restrict__ ticket, uint32_t num_tiles,
                                                        uint32_t* __restrict__ clear, uint32_t n_clear) {
     __shared__ uint32_t s_tile, s_prefix;
-    __shared__ uint32_t s_w[4];
+    __shared__ uint32_t s_tot[kScanRows][4];  // wave totals of each row
     const uint32_t tid = threadIdx.x;
     if (tid == 0) {
         const uint32_t t = atomicAdd(ticket, 1u);
@@ -662,8 +676,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* __restrict__ da
     const uint32_t tile = s_tile;
     const uint32_t base = tile * kScanTile;
     uint4 v[kScanRows];
-    uint32_t excl[kScanRows], rowOff[kScanRows];
-    uint32_t agg = 0;
+    uint32_t excl[kScanRows];  // entries of the tile before this thread's four in row r
+    const uint32_t lane = tid & 63u, wave = tid >> 6;
 #pragma unroll
     for (int r = 0; r < kScanRows; ++r) {
         const uint32_t idx = base + (uint32_t)r * (kScanThreads * 4u) + tid * 4u;
@@ -675,11 +689,22 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* __restrict__ da
             v[r].z = idx + 2u < n ? data[idx + 2] : 0u;
             v[r].w = 0u;
         }
+    }
+    // Every row's wave scans at once, then one barrier for the wave totals.
+#pragma unroll
+    for (int r = 0; r < kScanRows; ++r) {
         const uint32_t sum = v[r].x + v[r].y + v[r].z + v[r].w;
-        uint32_t rowTotal;
-        excl[r] = block_exclusive_scan(sum, rowTotal, s_w);
-        rowOff[r] = agg;
-        agg += rowTotal;
+        const uint32_t inc = wave_inclusive_scan_dpp(sum);
+        if (lane == 63u) s_tot[r][wave] = inc;
+        excl[r] = inc - sum;
+    }
+    __syncthreads();
+    uint32_t agg = 0;
+#pragma unroll
+    for (int r = 0; r < kScanRows; ++r) {
+        const uint32_t t0 = s_tot[r][0], t1 = s_tot[r][1], t2 = s_tot[r][2], t3 = s_tot[r][3];
+        excl[r] += agg + (wave > 0u ? t0 : 0u) + (wave > 1u ? t1 : 0u) + (wave > 2u ? t2 : 0u);
+        agg += t0 + t1 + t2 + t3;
     }
     if (tid < 64u) {
         // Look-back by the first wave, 64 predecessors at a time: lane l reads
@@ -726,7 +751,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* __restrict__ da
     for (int r = 0; r < kScanRows; ++r) {
         const uint32_t idx = base + (uint32_t)r * (kScanThreads * 4u) + tid * 4u;
         uint4 o;
-        o.x = prefix + rowOff[r] + excl[r] + v[r].x;
+        o.x = prefix + excl[r] + v[r].x;
         o.y = o.x + v[r].y;
         o.z = o.y + v[r].z;
         o.w = o.z + v[r].w;
