@@ -616,10 +616,10 @@ __global__ __launch_bounds__(kBlock) void k_force_runs(const BlockDesc* __restri
 // no memset is needed between steps and the launch can live in a hipGraph.
 constexpr int kScanThreads = 256;
 #ifndef SPH_SCAN_ROWS
-#define SPH_SCAN_ROWS 8
+#define SPH_SCAN_ROWS 16
 #endif
 constexpr int kScanRows = SPH_SCAN_ROWS;
-constexpr uint32_t kScanTile = kScanThreads * 4u * kScanRows;  // 8192 entries per tile
+constexpr uint32_t kScanTile = kScanThreads * 4u * kScanRows;  // 16384 entries per tile
 constexpr unsigned long long kStatAgg = 1ull << 32;
 constexpr unsigned long long kStatIncl = 2ull << 32;
 
