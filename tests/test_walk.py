@@ -17,3 +17,20 @@ def test_walk_visits_reference_buckets():
     r = subprocess.run([WALK_BIN], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "0 mismatches" in r.stdout
+
+
+def test_candidate_layout_positions():
+    """The cell-wave candidate layout (sph_cellwave.hpp: kSlots, slot_bucket) puts
+    slot l % kSlots of bucket l / kSlots at position l of four 64-lane rounds, with
+    l / 9 computed as (57 l) >> 9: every position below 27 kSlots maps to one
+    (bucket, slot) pair, bucket < 27, and the 27 buckets fit the 256 positions."""
+    import re
+    hdr = os.path.join(os.path.dirname(__file__), "..", "sphfluidsimulation_amd", "csrc", "sph_cellwave.hpp")
+    k = int(re.search(r"#define SPH_CW_SLOTS (\d+)", open(hdr).read()).group(1))
+    assert k in (8, 9) and 27 * k <= 256
+    seen = set()
+    for pos in range(27 * k):
+        b = pos >> 3 if k == 8 else (pos * 57) >> 9
+        assert b == pos // k and b < 27
+        seen.add((b, pos - k * b))
+    assert len(seen) == 27 * k and all(s < k for _, s in seen)
