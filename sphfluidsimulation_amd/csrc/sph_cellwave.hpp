@@ -42,9 +42,8 @@ constexpr uint32_t kCwMaskWords = 8;      // 4 rounds x 64 bits per particle
 #endif
 constexpr uint32_t kSlots = SPH_CW_SLOTS;
 static_assert(kSlots == 8 || kSlots == 9, "27 buckets of kSlots positions must fit four rounds of 64");
-// Bucket of layout position l < 27 kSlots (l / 9 as (57 l) >> 9, exact below 243;
-// a 24-bit multiply, full rate where a 32-bit one is quarter rate).
-__device__ inline uint32_t slot_bucket(uint32_t l) { return kSlots == 8 ? l >> 3 : __umul24(l, 57u) >> 9; }
+// Bucket of layout position l < 27 kSlots (l / 9 as (57 l) >> 9, exact below 243).
+__device__ inline uint32_t slot_bucket(uint32_t l) { return kSlots == 8 ? l >> 3 : (l * 57u) >> 9; }
 
 // Per-wave scratch: bucket start and count of the 27 neighbour buckets of up to
 // kGroupBatch groups.
