@@ -183,7 +183,10 @@ def main():
     if args.autotune:
         chosen = {e.autotune(dt, 4) for e in engines}
         args.walk = "/".join(sorted(chosen))
-        log(f"[bench] autotune chose {args.walk}")
+        handoff = {e.neighbours for e in engines if e.neighbours}
+        if handoff:
+            args.neighbours = "/".join(sorted(handoff))
+        log(f"[bench] autotune chose {args.walk} ({args.neighbours})")
     n_total = sum(e.n for e in engines)
     M_total = sum(e.hash_map_size(s) for e in engines for s in range(e.n_scenes))
     Ms_total = sum(sc.n_static for sc in scene_list)

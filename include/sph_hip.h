@@ -226,10 +226,14 @@ int sph_stage_times(sph_handle h, double* ms, uint64_t* launches);
 
 /* Autotuner with no counterpart in the reference (whose engines have one fixed
  * neighbour loop, SPHFunctions.cpp:397-404): times every neighbour-pass variant
- * (cell-group waves, Morton runs, LDS bucket tiles) for 2 warm-up steps plus
- * trial_steps (rounded up to an even count, at least 2) steps of dt on the
- * current state, keeps the fastest for later updates and writes its walk flag
- * (0, SPH_FLAG_RUN_WALK or SPH_FLAG_TILE_WALK) to *chosen. The state, the
+ * (cell-group waves with neighbour lists and with neighbour masks, Morton runs,
+ * LDS bucket tiles) for 2 warm-up steps plus trial_steps (rounded up to an even
+ * count, at least 2) steps of dt on the current state, keeps the fastest for
+ * later updates (the configured variant unless another is more than 3% faster)
+ * and writes its flags to *chosen: the walk flag (0,
+ * SPH_FLAG_RUN_WALK or SPH_FLAG_TILE_WALK), with SPH_FLAG_CW_LISTS or
+ * SPH_FLAG_CW_MASKS added for the cell-group waves. It allocates whichever of
+ * the lists and masks the engine lacks and keeps both. The state, the
  * simulation time and the results of later updates (within the float tolerance
  * of the variants' summation orders) are as if it had not been called: each
  * trial runs from a device copy of the state. Synchronous; allocates scratch

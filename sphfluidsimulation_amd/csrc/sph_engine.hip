@@ -1151,6 +1151,25 @@ int sph_stage_times(sph_handle h, double* ms, uint64_t* launches) {
     return SPH_OK;
 }
 
+namespace {
+// Both ways for the cell-wave walk to hand neighbours to the force pass (lists,
+// or masks with the crowded buckets' record), so the autotuner can time each:
+// allocates whichever the engine lacks and refreshes the density pass's output
+// pointers.
+int ensure_cw_handoffs(sph_engine* h) {
+    int rc;
+    if (!h->nbr) {
+        if ((rc = dalloc(h, &h->nbr, (uint64_t)std::max<uint32_t>(h->numBlocks, 1) * kWaves * kListCap * 64u)))
+            return rc;
+        if ((rc = dalloc(h, &h->nbrCount, std::max<uint64_t>(h->N, 1)))) return rc;
+    }
+    if (!h->extraMask && (rc = dalloc(h, &h->extraMask, std::max<uint64_t>(h->N, 1)))) return rc;
+    const DensityOut o{h->heavy, h->nbrCount, h->mask, h->extraMask, h->wallSum, h->wallZ, h->dens};
+    SPH_HIP(h, hipMemcpy(h->densOut, &o, sizeof(o), hipMemcpyHostToDevice));
+    return SPH_OK;
+}
+}  // namespace
+
 int sph_autotune(sph_handle h, float dt, uint32_t trial_steps, uint32_t* chosen) {
     if (!valid(h)) return SPH_E_INVALID;
     if (!h->initialized) {
@@ -1158,11 +1177,19 @@ int sph_autotune(sph_handle h, float dt, uint32_t trial_steps, uint32_t* chosen)
         return SPH_E_STATE;
     }
     const uint32_t walkBits = SPH_FLAG_CELL_WALK | SPH_FLAG_RUN_WALK | SPH_FLAG_TILE_WALK;
+    const uint32_t handoffBits = SPH_FLAG_CW_LISTS | SPH_FLAG_CW_MASKS;
+    auto handoff = [&](uint32_t walk, bool lists) {
+        return walk == 0u ? (lists ? SPH_FLAG_CW_LISTS : SPH_FLAG_CW_MASKS) : 0u;
+    };
     if (h->N == 0 || !h->mask) {  // nothing to time, or the engine was set up for the cell walk only
-        if (chosen) *chosen = h->flags & walkBits;
+        if (chosen) *chosen = (h->flags & walkBits) | handoff(h->flags & walkBits, h->cwList);
         return SPH_OK;
     }
     SPH_HIP(h, hipSetDevice(h->device));
+    {
+        const int arc = ensure_cw_handoffs(h);
+        if (arc) return arc;
+    }
     const uint32_t warm = 2, timed = std::max<uint32_t>(2u, (trial_steps + 1u) & ~1u);  // even: same parity after
     const uint64_t N = h->N, L = h->L;
     const int c = h->cur;
@@ -1214,13 +1241,28 @@ int sph_autotune(sph_handle h, float dt, uint32_t trial_steps, uint32_t* chosen)
         h->err = std::string("sph_autotune: ") + hipGetErrorString(e);
         return finish(SPH_E_HIP);
     }
-    const uint32_t candidates[3] = {0u, SPH_FLAG_RUN_WALK, SPH_FLAG_TILE_WALK};
-    const uint32_t base = h->flags & ~walkBits;
+    // The engine's configured variant first, then the others among the cell-wave
+    // walk with lists and with masks, the Morton runs and the bucket tiles. A
+    // variant replaces the best so far only when it is more than 3% faster: the
+    // trial steps are few, and the configured choice reflects the whole run
+    // (the lists' advantage at 1M grows as the fluid compresses).
+    uint32_t candidates[4] = {0u, 0u, SPH_FLAG_RUN_WALK, SPH_FLAG_TILE_WALK};
+    bool candLists[4] = {true, false, false, false};
+    {
+        const uint32_t cw = h->flags & walkBits;
+        int cur = cw == SPH_FLAG_RUN_WALK ? 2 : cw == SPH_FLAG_TILE_WALK ? 3 : (h->cwList ? 0 : 1);
+        std::swap(candidates[0], candidates[cur]);
+        std::swap(candLists[0], candLists[cur]);
+    }
+    const uint32_t base = h->flags & ~(walkBits | handoffBits);
     uint32_t best = h->flags & walkBits;
+    bool bestLists = h->cwList;
     float bestMs = 0.f;
     bool first = true;
-    for (uint32_t w : candidates) {
+    for (int ci = 0; ci < 4; ++ci) {
+        const uint32_t w = candidates[ci];
         h->flags = base | w;
+        if (w == 0u) h->cwList = candLists[ci];
         for (uint32_t s = 0; s < warm + timed; ++s) {
             if (s == warm && (e = hipEventRecord(e0, h->stream)) != hipSuccess) break;
             if ((rc = enqueue_step(h, dt, false))) return finish(rc);
@@ -1233,17 +1275,20 @@ int sph_autotune(sph_handle h, float dt, uint32_t trial_steps, uint32_t* chosen)
         h->cur = c;
         if (e == hipSuccess) e = restore();
         if (e != hipSuccess) {
-            h->flags = base | best;
+            h->flags = base | best | handoff(best, bestLists);
+            h->cwList = bestLists;
             h->err = std::string("sph_autotune: ") + hipGetErrorString(e);
             return finish(SPH_E_HIP);
         }
-        if (first || ms < bestMs) {
+        if (first || ms < 0.97f * bestMs) {
             best = w;
+            if (w == 0u) bestLists = candLists[ci];
             bestMs = ms;
             first = false;
         }
     }
-    h->flags = base | best;
+    h->cwList = bestLists;
+    h->flags = base | best | handoff(best, bestLists);
     for (int p = 0; p < 2; ++p)  // captured steps may hold another walk's kernels
         if (h->graph[p]) {
             (void)hipGraphExecDestroy(h->graph[p]);
@@ -1255,7 +1300,7 @@ int sph_autotune(sph_handle h, float dt, uint32_t trial_steps, uint32_t* chosen)
         h->err = std::string("sph_autotune: ") + hipGetErrorString(e);
         return finish(SPH_E_HIP);
     }
-    if (chosen) *chosen = best;
+    if (chosen) *chosen = best | handoff(best, bestLists);
     return finish(SPH_OK);
 }
 

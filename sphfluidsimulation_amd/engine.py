@@ -45,6 +45,7 @@ class SimulationEngineHIP:
             raise _lib.SPHError(rc, "sph_create failed (no HIP device?)")
         self._h = h
         self._scene_sizes = []  # (n, n_static, M) per scene
+        self.neighbours = None  # the cell-wave hand-off the last autotune chose ("lists" or "masks")
 
     # -- lifetime ---------------------------------------------------------
     def close(self):
@@ -210,11 +211,18 @@ class SimulationEngineHIP:
                   _lib.SPH_FLAG_CELL_WALK: "cells"}
 
     def autotune(self, dt: float, trial_steps: int = 4) -> str:
-        """Time every neighbour-pass variant on the current state, keep the
-        fastest; the state is left as it was. Returns the chosen walk's name."""
+        """Time every neighbour-pass variant on the current state (the cell-wave
+        walk with neighbour lists and with masks, the Morton runs, the bucket
+        tiles), keep the fastest; the state is left as it was. Returns the
+        chosen walk's name; for the cell-wave walk, `neighbours` then says
+        "lists" or "masks"."""
         chosen = ctypes.c_uint32(0)
         self._check(self._lib.sph_autotune(self._h, ctypes.c_float(dt), int(trial_steps), ctypes.byref(chosen)))
-        return self.WALK_NAMES[int(chosen.value)]
+        c = int(chosen.value)
+        walk = c & (_lib.SPH_FLAG_CELL_WALK | _lib.SPH_FLAG_RUN_WALK | _lib.SPH_FLAG_TILE_WALK)
+        self.neighbours = ("lists" if c & _lib.SPH_FLAG_CW_LISTS else
+                           "masks" if c & _lib.SPH_FLAG_CW_MASKS else None)
+        return self.WALK_NAMES[walk]
 
     # -- asynchronous readback (RenderableGPUParticleBufferManager.cpp:190-270) --
     def readback_enqueue(self) -> int:

@@ -443,8 +443,9 @@ def test_autotune_leaves_state_unchanged(s16k):
     starts0 = e.read_cell_starts()[0]
     t0 = e.GetSimulationTime()
     walk = e.autotune(1e-3, 4)
-    say(f"autotune chose {walk}")
+    say(f"autotune chose {walk} ({e.neighbours})")
     assert walk in ("cellwave", "runs", "tile")
+    assert (e.neighbours in ("lists", "masks")) == (walk == "cellwave")
     pos1, vel1, p1 = e.read()
     np.testing.assert_array_equal(pos1, pos0)
     np.testing.assert_array_equal(vel1, vel0)
