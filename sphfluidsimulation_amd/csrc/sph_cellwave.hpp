@@ -517,6 +517,29 @@ struct DensityOut {
     float4* wallSum;      // walls' geometric sums
     float4* wallZ;        // coincident walls' w1
     float* dens;          // density (debug recording)
+    float4* visc;         // viscosity sums over the dynamic neighbours (SPH_CW_VISC)
+};
+
+// List mode: the density pass, which visits every kept neighbour, also sums the
+// viscosity terms (v_j - v_i) w2(r) (SF:557-589; they need no pressure) in the
+// order the force pass would, and stores them; the force pass over the lists
+// then gathers only each neighbour's position and pressure (one 16-byte record
+// per list entry instead of two). Mask mode keeps the sums in the force pass:
+// the density pass's extra registers there would cost it a wave per SIMD.
+#ifndef SPH_CW_VISC_LISTS
+#define SPH_CW_VISC_LISTS 1
+#endif
+constexpr bool kViscLists = SPH_CW_VISC_LISTS != 0;
+// Density pass, list mode with the viscosity sums: two candidates per batch (the
+// velocities take the registers of a third).
+#ifndef SPH_CW_BATCH_DENSITY_LIST
+#define SPH_CW_BATCH_DENSITY_LIST 2
+#endif
+constexpr uint32_t kBatchDensityList = kViscLists ? SPH_CW_BATCH_DENSITY_LIST : kBatchDensity;
+template <bool kMixed>
+struct ViscSums {
+    Acc<kMixed> x, y, z;
+    float4 vi;
 };
 
 // kList: record neighbour lists for k_force_cw; else the neighbour masks and
@@ -527,6 +550,7 @@ template <bool kStatic, bool kDebug, bool kMixed, bool kList>
 #endif
 __global__ __launch_bounds__(kBlock, kMixed ? 6 : SPH_CW_DENSITY_WAVES) void k_density_cw(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
                                                        const SceneDev* __restrict__ scenes, float4* pos,
+                                                       const float4* __restrict__ vel,
                                                        const uint32_t* __restrict__ tab,
                                                        const float4* __restrict__ spos,
                                                        const uint32_t* __restrict__ stab,
@@ -593,12 +617,16 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : SPH_CW_DENSITY_WAVES) void k_d
         }
         pendHits = 0u;
     };
-    auto dn = [&](const auto& j) -> uint32_t {
+    constexpr bool kVisc = kList && kViscLists;
+    auto dn = [&](const auto& j, ViscSums<kMixed>& vs) -> uint32_t {
         constexpr uint32_t B = sizeof(j) / sizeof(j[0]);
         static_assert(B <= 4, "at most four pending candidates");
-        float4 xj[B];
+        float4 xj[B], vj[B];
 #pragma unroll
-        for (uint32_t u = 0; u < B; ++u) xj[u] = ld16(pos, j[u]);
+        for (uint32_t u = 0; u < B; ++u) {
+            xj[u] = ld16(pos, j[u]);
+            if constexpr (kVisc) vj[u] = ld16(vel, j[u]);
+        }
         flush();
         uint32_t hits = 0u;
 #pragma unroll
@@ -608,7 +636,15 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : SPH_CW_DENSITY_WAVES) void k_d
             const float r = __builtin_amdgcn_sqrtf(r2);
             const float q = h - r;
             const bool ok = within_h2(r2, h2) && j[u] != i;
-            sumDyn.add((ok && inside_h(r, h)) ? q * q * q : 0.f);
+            const bool in = inside_h(r, h);
+            sumDyn.add((ok && in) ? q * q * q : 0.f);
+            if constexpr (kVisc) {
+                // ForceSums::pair's viscosity terms, unchanged.
+                const float w2 = ok && in ? 6.0f * q : 0.f;
+                vs.x.add((vj[u].x - vs.vi.x) * w2);
+                vs.y.add((vj[u].y - vs.vi.y) * w2);
+                vs.z.add((vj[u].z - vs.vi.z) * w2);
+            }
             hits |= ok ? 1u << u : 0u;
             if constexpr (kList) pend[u] = j[u];
         }
@@ -711,7 +747,10 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : SPH_CW_DENSITY_WAVES) void k_d
             uint64_t m[4];
 #pragma unroll
             for (uint32_t r = 0; r < 4; ++r) m[r] = ((uint64_t)m8.w[2 * r + 1] << 32) | m8.w[2 * r];
-            walk_mask<kBatchDensity, false>(m, gt.start[wave][slot], i, dn);
+            ViscSums<kMixed> vs;
+            if constexpr (kVisc) vs.vi = ld16(vel, i);
+            auto dnv = [&](const auto& j) { return dn(j, vs); };
+            walk_mask<kList ? kBatchDensityList : kBatchDensity, false>(m, gt.start[wave][slot], i, dnv);
             // Crowded buckets: their particles past the kSlots-th, a batch at a time.
             // Mask mode: extraMask[i] bit k says extra particle k (enumerated bucket
             // by bucket in visit order) is kept; all ones when there are more than
@@ -726,7 +765,7 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : SPH_CW_DENSITY_WAVES) void k_d
                     uint32_t j[kBatchCrowded];
 #pragma unroll
                     for (uint32_t u = 0; u < kBatchCrowded; ++u) j[u] = jj + u < e ? jj + u : i;
-                    const uint32_t hits = dn(j);  // padded entries (j = i) are never kept
+                    const uint32_t hits = dn(j, vs);  // padded entries (j = i) are never kept
                     if (!kList) {
                         if (k < kExtraBits) bits |= (uint64_t)hits << k;
                         k += min(kBatchCrowded, e - jj);
@@ -735,6 +774,7 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : SPH_CW_DENSITY_WAVES) void k_d
             }
             flush();
             if (!kList && anyCrowded) out->extraMask[i] = k > kExtraBits ? ~0ull : bits;
+            if constexpr (kVisc) out->visc[i] = make_float4(vs.x.s, vs.y.s, vs.z.s, 0.f);
         });
     if (kStatic && ballot64(valid && (walls & ~kPoisonBit) != 0u) != 0ull && lane == 0u) out->heavy[blk] = 1u;
     if (!valid) return;
@@ -763,7 +803,7 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : SPH_CW_DENSITY_WAVES) void k_d
 // Force pass of the cell-wave walk: the neighbour sums of one particle
 // (SF:557-589) and, in force_finish, the walls, the coincident-pair direction,
 // the force and the integration with the new key and histogram (SF:590-656).
-template <bool kMixed>
+template <bool kMixed, bool kVisc>
 struct ForceSums {
     float4 xi, vi;   // the particle (pressure in xi.w)
     uint32_t i;      // its index
@@ -801,9 +841,46 @@ struct ForceSums {
         fvy.add((vj.y - vi.y) * w2);
         fvz.add((vj.z - vi.z) * w2);
     }
+    // The density pass's viscosity sums for this particle (SPH_CW_VISC).
+    __device__ void take_visc(const float4* __restrict__ visc) {
+        if constexpr (kVisc) {
+            const float4 v = ld16(visc, i);
+            fvx.s = v.x;
+            fvy.s = v.y;
+            fvz.s = v.z;
+        }
+    }
+    // The pressure terms of pair alone (the viscosity sums come from the density
+    // pass, SPH_CW_VISC).
+    __device__ void pair_p(uint32_t j, const float4 xj) {
+        const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const bool ok = j != i && within_h2(r2, h2);
+        const float r = __builtin_amdgcn_sqrtf(r2);
+        const bool zero = r == 0.f;  // r2 == 0 implies r == 0
+        const float inv = zero ? 0.f : __builtin_amdgcn_rcpf(r);
+        const float q = h - r;
+        const bool in = inside_h(r, h);
+        const float w1 = in ? -3.0f * q * q : 0.f;
+        const float sp = ok ? (xi.w + xj.w) * w1 : 0.f;
+        fpx.add((dx * inv) * sp);
+        fpy.add((dy * inv) * sp);
+        fpz.add((dz * inv) * sp);
+        zeroSp += ok && zero ? sp : 0.f;
+        anyZero |= ok && zero;
+    }
     // A batch of candidates, their loads issued together.
     template <class J>
     __device__ void batch(const float4* __restrict__ pos, const float4* __restrict__ vel, const J& j) {
+        if constexpr (kVisc) {
+            constexpr uint32_t B = sizeof(j) / sizeof(j[0]);
+            float4 xj[B];
+#pragma unroll
+            for (uint32_t u = 0; u < B; ++u) xj[u] = ld16(pos, j[u]);
+#pragma unroll
+            for (uint32_t u = 0; u < B; ++u) pair_p(j[u], xj[u]);
+            return;
+        }
         constexpr uint32_t B = sizeof(j) / sizeof(j[0]);
         float4 xj[B], vj[B];
 #pragma unroll
@@ -820,8 +897,8 @@ struct ForceSums {
 // only; the active lanes of the wave are a prefix of it). `poisoned`: the
 // density pass skipped the NaN pile for it, so its acceleration is NaN, as the
 // pile makes it.
-template <bool kStatic, bool kDebug, bool kTri, bool kMixed>
-__device__ inline void force_finish(ForceSums<kMixed>& f, const SceneDev& S, uint32_t scene, uint32_t lane,
+template <bool kStatic, bool kDebug, bool kTri, bool kMixed, bool kVisc>
+__device__ inline void force_finish(ForceSums<kMixed, kVisc>& f, const SceneDev& S, uint32_t scene, uint32_t lane,
                                     bool poisoned, const float4* __restrict__ wallSum,
                                     const float4* __restrict__ wallZ, const uint32_t* __restrict__ ids,
                                     const float* __restrict__ tris, float dt, float4* __restrict__ posT,
@@ -900,6 +977,7 @@ __global__ __launch_bounds__(kBlock) void k_force_cw(const BlockDesc* __restrict
                                                      const uint32_t* __restrict__ tab,
                                                      const float4* __restrict__ wallSum,
                                                      const float4* __restrict__ wallZ,
+                                                     const float4* __restrict__ visc,
                                                      const uint32_t* __restrict__ nbr,
                                                      const uint32_t* __restrict__ nbrCount,
                                                      uint32_t* __restrict__ nanNext,
@@ -913,7 +991,7 @@ __global__ __launch_bounds__(kBlock) void k_force_cw(const BlockDesc* __restrict
     const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
     const uint32_t i = bd.begin + threadIdx.x;
     if (i >= bd.end) return;  // the active lanes of a wave stay a prefix
-    ForceSums<kMixed> f;
+    ForceSums<kMixed, kViscLists> f;  // the density pass holds the viscosity sums
     f.xi = ld16(pos, i);
     f.vi = ld16(vel, i);
     f.i = i;
@@ -942,6 +1020,7 @@ __global__ __launch_bounds__(kBlock) void k_force_cw(const BlockDesc* __restrict
                       }
                   });
     }
+    f.take_visc(visc);
     force_finish<kStatic, kDebug, kTri, kMixed>(f, S, bd.scene, lane, n == kListPoison, wallSum, wallZ, ids, tris,
                                                 dt, posT, velT, counts, accOut, nanNext);
 }
@@ -956,7 +1035,8 @@ template <bool kStatic, bool kDebug, bool kTri, bool kMixed>
 __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw_mask(
     const BlockDesc* __restrict__ blocks, uint32_t numBlocks, const SceneDev* __restrict__ scenes,
     const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ tab,
-    const float4* __restrict__ wallSum, const float4* __restrict__ wallZ, const uint32_t* __restrict__ mask,
+    const float4* __restrict__ wallSum, const float4* __restrict__ wallZ, const float4* __restrict__ visc,
+    const uint32_t* __restrict__ mask,
     const uint64_t* __restrict__ extraMask, const uint32_t* __restrict__ nanCnt, uint32_t* __restrict__ nanNext,
     const uint32_t* __restrict__ ids, const float* __restrict__ tris, float dt, float4* __restrict__ posT,
     float4* __restrict__ velT, uint32_t* __restrict__ counts, float4* __restrict__ accOut) {
@@ -972,7 +1052,7 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw_mask(
     const uint32_t i = wbegin + lane;
     const bool valid = lane < wcount;
     const uint32_t iv = valid ? i : wbegin;
-    ForceSums<kMixed> f;
+    ForceSums<kMixed, false> f;
     f.xi = ld16(pos, iv);
     f.vi = ld16(vel, iv);
     f.i = i;
@@ -1018,6 +1098,7 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw_mask(
             }
         });
     if (!valid) return;
+    f.take_visc(visc);
     force_finish<kStatic, kDebug, kTri, kMixed>(f, S, bd.scene, lane, (noWalls & kPoisonBit) != 0u, wallSum, wallZ,
                                                 ids, tris, dt, posT, velT, counts, accOut, nanNext);
 }

@@ -66,7 +66,8 @@ struct sph_engine {
     uint32_t* tileOrder = nullptr;  // cell-wave launch order (tile_order_chunk), numBlocks entries
     uint32_t* heavy = nullptr;      // per block: particles near walls in the last density pass
     float4* wallSum = nullptr;      // cell-wave: the walls' geometric sums of each particle, for the force pass
-    float4* wallZ = nullptr;        // and the coincident-pair weight where one occurred
+    float4* wallZ = nullptr;
+    float4* visc = nullptr;         // cell-wave: the density pass's viscosity sums (SPH_CW_VISC)        // and the coincident-pair weight where one occurred
     uint64_t wallWords = 0;
     uint32_t* staticScene = nullptr;
     float* tris = nullptr;
@@ -170,6 +171,7 @@ void release(sph_engine* h) {
     dfree(h->heavy);
     dfree(h->wallSum);
     dfree(h->wallZ);
+    dfree(h->visc);
     h->wallWords = 0;
     dfree(h->staticScene);
     dfree(h->tris);
@@ -309,7 +311,8 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     if (walk != Walk::Cells) {
 #define SPH_DENS_CW(S, D, X, L)                                                                                \
         hipLaunchKernelGGL((k_density_cw<S, D, X, L>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,   \
-                           h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->wallMap, h->tileOrder, h->nbr,  \
+                           h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->wallMap, h->tileOrder,  \
+                           h->nbr,                                                                             \
                            nanNow, h->densOut)
 #define SPH_DENS(S, D)                                                                                         \
     if (walk == Walk::CellWave && mixed && h->cwList)                                                          \
@@ -363,11 +366,13 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     if ((rc = begin(SPH_STAGE_FORCE))) return rc;
 #define SPH_FORCE_LIST(S, D, T, X)                                                                                \
         hipLaunchKernelGGL((k_force_cw<S, D, T, X>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,       \
-                           h->dScenes, h->pos, h->vel, h->tab[c], h->wallSum, h->wallZ, h->nbr, h->nbrCount,      \
+                           h->dScenes, h->pos, h->vel, h->tab[c], h->wallSum, h->wallZ, h->visc, h->nbr,          \
+                           h->nbrCount,                                                                           \
                            nanNext, h->ids[c], h->tris, dt, h->posT, h->velT, h->tab[c ^ 1], h->acc)
 #define SPH_FORCE_MASK(S, D, T, X)                                                                                \
         hipLaunchKernelGGL((k_force_cw_mask<S, D, T, X>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,  \
-                           h->dScenes, h->pos, h->vel, h->tab[c], h->wallSum, h->wallZ, h->mask, h->extraMask,    \
+                           h->dScenes, h->pos, h->vel, h->tab[c], h->wallSum, h->wallZ, h->visc, h->mask,         \
+                           h->extraMask,                                                                          \
                            nanNow, nanNext, h->ids[c], h->tris, dt, h->posT, h->velT, h->tab[c ^ 1], h->acc)
 #define SPH_FORCE(S, D, T)                                                                                        \
     do {                                                                                                          \
@@ -696,6 +701,7 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
                   "every mask format uses 8 words per particle");
     if (!(flags & SPH_FLAG_CELL_WALK)) {
         SPH_ALLOC(h->mask, kMaskWords * std::max<uint64_t>(N, 1));
+        if (kViscLists) SPH_ALLOC(h->visc, std::max<uint64_t>(N, 1));
         // Lists while they stay in the caches (4 B per neighbour), else masks (32 B per particle).
         h->cwList = (flags & SPH_FLAG_CW_LISTS) || (!(flags & SPH_FLAG_CW_MASKS) && N <= SPH_CW_LIST_MAX);
         if (h->cwList) {
@@ -736,7 +742,7 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
         SPH_HIP(h, hipMemset(h->heavy, 0, sizeof(uint32_t) * identity.size()));
     }
     if (h->mask) {
-        const DensityOut o{h->heavy, h->nbrCount, h->mask, h->extraMask, h->wallSum, h->wallZ, h->dens};
+        const DensityOut o{h->heavy, h->nbrCount, h->mask, h->extraMask, h->wallSum, h->wallZ, h->dens, h->visc};
         if ((rc = dalloc(h, &h->densOut, 1))) return rc;
         SPH_HIP(h, hipMemcpy(h->densOut, &o, sizeof(o), hipMemcpyHostToDevice));
     }
@@ -1164,7 +1170,7 @@ int ensure_cw_handoffs(sph_engine* h) {
         if ((rc = dalloc(h, &h->nbrCount, std::max<uint64_t>(h->N, 1)))) return rc;
     }
     if (!h->extraMask && (rc = dalloc(h, &h->extraMask, std::max<uint64_t>(h->N, 1)))) return rc;
-    const DensityOut o{h->heavy, h->nbrCount, h->mask, h->extraMask, h->wallSum, h->wallZ, h->dens};
+    const DensityOut o{h->heavy, h->nbrCount, h->mask, h->extraMask, h->wallSum, h->wallZ, h->dens, h->visc};
     SPH_HIP(h, hipMemcpy(h->densOut, &o, sizeof(o), hipMemcpyHostToDevice));
     return SPH_OK;
 }
