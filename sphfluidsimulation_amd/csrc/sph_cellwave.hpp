@@ -517,7 +517,7 @@ struct DensityOut {
     float4* wallSum;      // walls' geometric sums
     float4* wallZ;        // coincident walls' w1
     float* dens;          // density (debug recording)
-    float4* visc;         // viscosity sums over the dynamic neighbours (SPH_CW_VISC)
+    float4* visc;         // viscosity sums over the dynamic neighbours (list mode)
 };
 
 // List mode: the density pass, which visits every kept neighbour, also sums the
@@ -803,6 +803,8 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : SPH_CW_DENSITY_WAVES) void k_d
 // Force pass of the cell-wave walk: the neighbour sums of one particle
 // (SF:557-589) and, in force_finish, the walls, the coincident-pair direction,
 // the force and the integration with the new key and histogram (SF:590-656).
+// kVisc: the density pass summed the viscosity terms (list mode), so the
+// neighbour walk takes only positions with pressure and adds the pressure terms.
 template <bool kMixed, bool kVisc>
 struct ForceSums {
     float4 xi, vi;   // the particle (pressure in xi.w)
@@ -841,7 +843,7 @@ struct ForceSums {
         fvy.add((vj.y - vi.y) * w2);
         fvz.add((vj.z - vi.z) * w2);
     }
-    // The density pass's viscosity sums for this particle (SPH_CW_VISC).
+    // The density pass's viscosity sums for this particle (kVisc).
     __device__ void take_visc(const float4* __restrict__ visc) {
         if constexpr (kVisc) {
             const float4 v = ld16(visc, i);
@@ -850,8 +852,8 @@ struct ForceSums {
             fvz.s = v.z;
         }
     }
-    // The pressure terms of pair alone (the viscosity sums come from the density
-    // pass, SPH_CW_VISC).
+    // The pressure terms of pair alone (kVisc: the viscosity sums come from the
+    // density pass).
     __device__ void pair_p(uint32_t j, const float4 xj) {
         const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
         const float r2 = dx * dx + dy * dy + dz * dz;

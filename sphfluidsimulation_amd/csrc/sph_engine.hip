@@ -67,7 +67,7 @@ struct sph_engine {
     uint32_t* heavy = nullptr;      // per block: particles near walls in the last density pass
     float4* wallSum = nullptr;      // cell-wave: the walls' geometric sums of each particle, for the force pass
     float4* wallZ = nullptr;
-    float4* visc = nullptr;         // cell-wave: the density pass's viscosity sums (SPH_CW_VISC)        // and the coincident-pair weight where one occurred
+    float4* visc = nullptr;         // cell-wave list mode: the density pass's viscosity sums        // and the coincident-pair weight where one occurred
     uint64_t wallWords = 0;
     uint32_t* staticScene = nullptr;
     float* tris = nullptr;
