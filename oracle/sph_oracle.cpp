@@ -35,9 +35,13 @@
 //   update_pressure  tests/test_oracle_engine.py: O(N^2) numpy density with the
 //                    27-cell bucket-visit rule (double visits on colliding
 //                    buckets), FP32 against FP64.
-//   update_dynamics  tests/test_oracle_engine.py: closed-form two-particle
-//                    system, free fall, wall force sign (|p_i|), zero static
-//                    viscosity, O(N^2) numpy forces, FP32 against FP64.
+//   update_dynamics  tests/test_oracle_engine.py: closed-form accelerations of a
+//                    particle pair (pressure and viscosity, p > 0 and p < 0) and
+//                    of a particle beside one wall (|p_i|, the wall summed once
+//                    per neighbour-cell visit), an all-pairs numpy acceleration
+//                    weighted by bucket-visit counts (hash sizes 2N, 7, 1000),
+//                    free fall, momentum balance, Galilean invariance, zero
+//                    static viscosity, FP32 against FP64.
 #include <algorithm>
 #include <atomic>
 #include <cfloat>

@@ -8,8 +8,10 @@ Checks used (none retypes the oracle's loops):
     neighbour's bucket among the 27 bucket visits of the particle, so colliding
     buckets (P5 of the survey: M not covering the Morton range) are double
     counted exactly as the reference's walk does;
-  * forces: physical special cases (free fall under symplectic Euler, a pressure
-    pair pushing apart, momentum balance of a symmetric cluster, Galilean
+  * forces: closed-form accelerations of a particle pair and of a particle beside
+    one wall, an all-pairs numpy acceleration weighted by the same bucket-visit
+    counts, and physical special cases (free fall under symplectic Euler, a
+    pressure pair pushing apart, momentum balance of a symmetric cluster, Galilean
     invariance of viscosity, repulsive walls with |p_i|, zero wall viscosity);
   * FP32 against FP64.
 """
@@ -72,10 +74,12 @@ def brute_density(params, pos, spos, M, Ms):
     return self_d + (dyn * m + st * m) * C
 
 
-def small_scene(n_side=12, seed=3, shift=(0.0, 0.0, 0.0)):
+def small_scene(n_side=12, seed=3, shift=(0.0, 0.0, 0.0), wall_gap=0.1):
+    """A jittered 12 x 6 x 12 block in the corner of a wall box whose planes lie
+    wall_gap below the block's first lattice layer on every axis."""
     rng = np.random.default_rng(seed)
     pos = scenes.lattice(n_side, n_side // 2, n_side, 0.05, offset=shift, jitter_frac=0.2, rng=rng)
-    walls = scenes.box_shell((1.2, 0.6, 1.2), 0.05, offset=np.float32(shift) - np.float32([0.1, 0.1, 0.1]))
+    walls = scenes.box_shell((1.2, 0.6, 1.2), 0.05, offset=np.float32(shift) - np.float32([wall_gap] * 3))
     return scenes.Scene(scenes.dam_break_params(), pos, np.zeros_like(pos), walls, name="small")
 
 
@@ -273,6 +277,137 @@ def test_wall_force_uses_abs_pressure_and_no_viscosity():
         res.append(o.read_debug()[1][0])
     assert res[0][1] > 0  # away from the wall below it
     np.testing.assert_array_equal(res[0], res[1])
+
+
+# ---------------------------------------------------------------------------
+# force magnitudes: closed forms of small systems and an all-pairs brute force
+#
+# SPHFunctions.cpp:544-638 in closed form. With r the distance, dir = (x_j - x_i)/r,
+# D1 = -3 (h - r)^2, D2 = 6 (h - r), rho_i = p_i / k + rho0 (SF:544):
+#   a_i = [ m C / (2 rho_i) * sum_j dir (p_i + p_j) D1          (pressure, SF:585, 626)
+#         + mu m C * sum_j (v_j - v_i) D2                       (viscosity, SF:588, 627)
+#         + m C / (2 rho_i) * sum_w dir |p_i| D1 ] / rho_i + g  (walls, SF:619, 628; x0 viscosity SF:629)
+# Each test below evaluates that for a system small enough to write out by hand, so a
+# scale factor dropped or changed in the oracle (m/rho for m/(2 rho), mu m for mu m C,
+# a wall factor other than 1) changes an acceleration by a factor of 2 or more.
+
+BIG_M = 1 << 15  # dynamic buckets covering the Morton codes of every cell used below: no collisions
+
+
+def _consts(p):
+    f = lambda v: float(np.float32(v))  # the engines take float32 parameters
+    h, m = f(p.maxInteractionDistance), f(p.particleMass)
+    return h, m, 15.0 / (3.1415 * h ** 6), f(p.gasConstant), f(p.restDensity), f(p.viscosity), \
+        np.array([f(p.gravityX), f(p.gravityY), f(p.gravityZ)])
+
+
+@pytest.mark.parametrize("rest_density", [10.0, 5000.0])  # p > 0 and p < 0
+def test_pair_acceleration_closed_form(rest_density):
+    """Two particles at distance r along x with velocities v0, v1. Each has density
+    m C (h^3 + (h - r)^3) and pressure p = k (rho - rho0), so
+      a_0 = -3 m C p (h - r)^2 / rho^2 * x + 6 mu m C (h - r) (v1 - v0) / rho + g
+    and a_1 the mirror image."""
+    p = params(restDensity=rest_density, viscosity=0.3)
+    h, m, C, k, rho0, mu, g = _consts(p)
+    x = np.float32([[0.5, 0.5, 0.5], [0.56, 0.5, 0.5]])
+    v = np.float32([[0.2, -0.1, 0.05], [-0.3, 0.4, 0.0]])
+    o = orc.Oracle(p, x, v, None, threads=1, fp64=True, hash_size=BIG_M, flags=orc.FLAG_RECORD_DEBUG)
+    o.update(1e-4, 1)
+    d, a = o.read_debug()
+    X, V = x.astype(np.float64), v.astype(np.float64)
+    r = X[1, 0] - X[0, 0]
+    rho = m * C * (h ** 3 + (h - r) ** 3)
+    pr = k * (rho - rho0)
+    np.testing.assert_allclose(d, rho, rtol=1e-13)
+    ex = np.array([1.0, 0.0, 0.0])
+    a0 = -3 * m * C * pr * (h - r) ** 2 / rho ** 2 * ex + 6 * mu * m * C * (h - r) * (V[1] - V[0]) / rho + g
+    a1 = 3 * m * C * pr * (h - r) ** 2 / rho ** 2 * ex + 6 * mu * m * C * (h - r) * (V[0] - V[1]) / rho + g
+    want = np.stack([a0, a1])
+    assert np.abs(want).max() > 10.0  # the forces dominate gravity
+    assert max_norm_rel(a, want) < 1e-12
+
+
+@pytest.mark.parametrize("rest_density", [10.0, 5000.0])  # p > 0 and p < 0
+def test_wall_acceleration_closed_form(rest_density):
+    """One particle and one wall particle at distance r. With one wall, M_s = N_s = 1,
+    so each of the particle's 27 neighbour cells maps to the wall's bucket and the wall
+    is summed 27 times (SPHFunctions.cpp:397-404, 596-599):
+      rho = m C (h^3 + 27 (h - r)^3),  p = k (rho - rho0),
+      a = 27 * m C / (2 rho) * |p| * (-3 (h - r)^2) * dir / rho + g,
+    which points away from the wall whatever the sign of p and does not depend on the
+    particle's velocity."""
+    p = params(restDensity=rest_density, viscosity=0.3)
+    h, m, C, k, rho0, mu, g = _consts(p)
+    x = np.float32([[0.5, 0.5, 0.5]])
+    wall = np.float32([[0.53, 0.46, 0.51]])
+    X, W = x.astype(np.float64)[0], wall.astype(np.float64)[0]
+    r = float(np.linalg.norm(W - X))
+    rho = m * C * (h ** 3 + 27 * (h - r) ** 3)
+    pr = k * (rho - rho0)
+    want = 27 * m * C / (2 * rho) * abs(pr) * (-3 * (h - r) ** 2) * (W - X) / r / rho + g
+    for vel in (np.float32([[0, 0, 0]]), np.float32([[0.3, -2.0, 0.1]])):
+        o = orc.Oracle(p, x, vel, wall, threads=1, fp64=True, flags=orc.FLAG_RECORD_DEBUG)
+        o.update(1e-4, 1)
+        d, a = o.read_debug()
+        assert d[0] == pytest.approx(rho, rel=1e-13)
+        assert max_norm_rel(a[0], want) < 1e-12
+
+
+def brute_accel(prm, pos, vel, spos, M, Ms):
+    """Acceleration of every particle from all pairs, each pair weighted by how many of
+    the particle's 27 bucket visits reach the other particle's bucket (the density of
+    brute_density supplies p_i and rho_i)."""
+    h, m, C, k, rho0, mu, g = _consts(prm)
+    rho = brute_density(prm, pos, spos, M, Ms)
+    pr = k * (rho - rho0)
+    rho_i = pr / k + rho0
+    k_dyn, cells = np_keys(pos, h, M)
+    P, V = pos.astype(np.float64), vel.astype(np.float64)
+    n = P.shape[0]
+    d = P[None, :, :] - P[:, None, :]  # d[i, j] = x_j - x_i
+    r = np.sqrt((d ** 2).sum(-1))
+    keep = (r * r <= h * h) & ~np.eye(n, dtype=bool)
+    cnt = np.where(keep, visit_counts(cells, M, k_dyn), 0)
+    rs = np.where(keep, r, 1.0)
+    w1 = cnt * -3.0 * (h - rs) ** 2
+    w2 = cnt * 6.0 * (h - rs)
+    Fp = ((w1 * (pr[:, None] + pr[None, :]))[:, :, None] * d / rs[:, :, None]).sum(1)
+    Fv = (w2[:, :, None] * (V[None, :, :] - V[:, None, :])).sum(1)
+    Fps = np.zeros_like(P)
+    if spos is not None and len(spos):
+        k_st, _ = np_keys(spos, h, Ms)
+        S = spos.astype(np.float64)
+        ds = S[None, :, :] - P[:, None, :]
+        q = np.sqrt((ds ** 2).sum(-1))
+        keep_s = q * q <= h * h
+        cs = np.where(keep_s, visit_counts(cells, Ms, k_st), 0)
+        qs = np.where(keep_s, q, 1.0)
+        Fps = ((cs * -3.0 * (h - qs) ** 2 * np.abs(pr)[:, None])[:, :, None] * ds / qs[:, :, None]).sum(1)
+    F = (Fp + Fps) * (m / (2 * rho_i) * C)[:, None] + Fv * (mu * m * C)
+    return F / rho_i[:, None] + g
+
+
+@pytest.mark.parametrize("hash_size", [0, 7, 1000])
+def test_accel_brute_force(hash_size):
+    """Pressure, viscosity and wall forces of a jittered block with random velocities
+    touching walls (0.04 below its first layer); hash sizes 7 and 1000 make buckets collide
+    (double visits)."""
+    sc = small_scene(wall_gap=0.04)
+    v = np.random.default_rng(11).normal(0, 0.3, size=sc.pos.shape).astype(np.float32)
+    o = orc.Oracle(sc.params, sc.pos, v, sc.static_pos, threads=2, fp64=True, hash_size=hash_size,
+                   flags=orc.FLAG_RECORD_DEBUG)
+    o.update(1e-3, 1)
+    _, a = o.read_debug()
+    want = brute_accel(sc.params, sc.pos, v, sc.static_pos, o.M, sc.n_static)
+    g = np.float32(sc.params.gravityY)
+    assert np.abs(want - [0, g, 0]).max() > 10 * abs(g)  # forces, not gravity, dominate
+    if hash_size == 0:  # the walls carry a large share of the force on the particles next to them
+        no_walls = brute_accel(sc.params, sc.pos, v, None, o.M, sc.n_static)
+        assert max_norm_rel(no_walls, want) > 0.2
+    assert max_norm_rel(a, want) < 1e-10
+    f = orc.Oracle(sc.params, sc.pos, v, sc.static_pos, threads=2, hash_size=hash_size, flags=orc.FLAG_RECORD_DEBUG)
+    f.update(1e-3, 1)
+    assert max_norm_rel(f.read_debug()[1], want) < 1e-5
 
 
 def test_fp32_tracks_fp64():
