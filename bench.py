@@ -33,8 +33,9 @@ CONFIGS = {
     "8m": ("gpu_8m", 5e-4, 500, {}),
     # 256 scenes advance together, each step replayed from one captured hipGraph.
     "batch256": ("batch_256", 1e-3, 500, {"graph": True}),
-    # The same scenes, one engine (and one launch per stage) per scene.
-    "batch256-plain": ("batch_256", 1e-3, 100, {}),
+    # The same scenes, one engine (and one launch per stage) per scene, over the
+    # same 500-step window.
+    "batch256-plain": ("batch_256", 1e-3, 500, {}),
     # 64 scenes in mixed precision.
     "batch64": ("batch_64_mixed", 1e-3, 200, {"graph": True, "mixed": True}),
 }
@@ -88,27 +89,64 @@ def load_traffic(kernel: str, workload: str):
     return None
 
 
-def cpu_baseline(scene, dt: float, budget_s: float, threads: int):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as orc
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
+
+def baseline_threads() -> int:
+    """Threads for the CPU baseline: the CPUs this process may run on (the
+    reference sizes its pool by std::thread::hardware_concurrency), capped by the
+    job's CPU share when the host states one in OMP_NUM_THREADS."""
+    try:
+        t = len(os.sched_getaffinity(0))
+    except AttributeError:
+        t = os.cpu_count() or 1
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        t = min(t, int(share))
+    return max(1, t)
+
+
+def time_oracle(orc, scene, dt: float, budget_s: float, threads: int):
+    """Particle-updates/s of the oracle on `scene` with `threads` threads: one
+    warm-up Update, then as many steps as fit the budget (2 .. 200)."""
     ref = orc.Oracle(scene.params, scene.pos, scene.vel, scene.static_pos, threads=threads)
     t0 = time.perf_counter()
-    ref.update(dt, 1)  # warm-up step
+    ref.update(dt, 1)
     t1 = time.perf_counter() - t0
     steps = max(2, min(200, int(budget_s / max(t1, 1e-6))))
     t0 = time.perf_counter()
     ref.update(dt, steps)
     el = time.perf_counter() - t0
     ref.close()
+    return scene.n * steps / el, steps, el
+
+
+def cpu_baseline(scene, dt: float, budget_s: float, threads: int):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    value, steps, el = time_oracle(orc, scene, dt, budget_s, threads)
+    one, steps1, el1 = time_oracle(orc, scene, dt, budget_s / 2, 1) if threads > 1 else (value, steps, el)
     return {
-        "value": scene.n * steps / el,
+        "value": value,
         "unit": "particle-updates/s",
         "cores": threads,
+        "threads": threads,
+        "value_1_thread": one,
+        "cpu_model": cpu_model(),
+        "host_cpus": os.cpu_count(),
         "kind": "port",
         "sample": f"{scene.name}: {steps} steps of {scene.n} particles after 1 warm-up step, "
-                  f"restated reference CPU engine (AoS, particleMap, serial scan, validators off), "
-                  f"{threads} threads, {el:.1f} s",
+                  f"restated reference CPU engine (AoS, particleMap, serial scan, validators off: a stated "
+                  f"deviation, the reference's debug validators run every step), {threads} threads, {el:.1f} s; "
+                  f"1 thread: {steps1} steps, {el1:.1f} s",
     }
 
 
@@ -130,7 +168,8 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="after initialize, let the engine time every walk and keep the fastest")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
+    ap.add_argument("--cpu-budget", type=float, default=10.0,
+                    help="seconds of CPU baseline work with all threads (half that again with one thread)")
     ap.add_argument("--profile-all", action="store_true", help="time every stage with events (default: dominant)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (HBM counter calibration) and exit")
@@ -304,8 +343,8 @@ def main():
         e.close()
 
     if not args.no_cpu_baseline and len(scene_list) == 1:
-        threads = min(os.cpu_count() or 1, 16)
-        log(f"[bench] CPU baseline: {threads} threads, ~{args.cpu_budget:.0f} s")
+        threads = baseline_threads()
+        log(f"[bench] CPU baseline: {threads} threads (host: {os.cpu_count()} CPUs), then 1 thread")
         result["cpu_baseline"] = cpu_baseline(scene_list[0], dt, args.cpu_budget, threads)
     else:
         result["cpu_baseline"] = None

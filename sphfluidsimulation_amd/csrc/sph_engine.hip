@@ -1200,6 +1200,8 @@ int sph_autotune(sph_handle h, float dt, uint32_t trial_steps, uint32_t* chosen)
     const uint64_t N = h->N, L = h->L;
     const int c = h->cur;
     const uint32_t savedProfiling = h->profiling;
+    const uint32_t savedFlags = h->flags;
+    const bool savedList = h->cwList;
     h->profiling = 0;
     float4 *bPos = nullptr, *bVel = nullptr;
     uint32_t *bIds = nullptr, *bTab = nullptr, *bNan = nullptr;
@@ -1265,13 +1267,24 @@ int sph_autotune(sph_handle h, float dt, uint32_t trial_steps, uint32_t* chosen)
     bool bestLists = h->cwList;
     float bestMs = 0.f;
     bool first = true;
+    // A trial that fails to enqueue leaves the engine as configured: the
+    // starting parity, the configured variant and the state of the snapshot.
+    auto abandon = [&](int code) {
+        h->cur = c;
+        h->flags = savedFlags;
+        h->cwList = savedList;
+        (void)hipStreamSynchronize(h->stream);
+        (void)restore();
+        (void)hipStreamSynchronize(h->stream);
+        return finish(code);
+    };
     for (int ci = 0; ci < 4; ++ci) {
         const uint32_t w = candidates[ci];
         h->flags = base | w;
         if (w == 0u) h->cwList = candLists[ci];
         for (uint32_t s = 0; s < warm + timed; ++s) {
             if (s == warm && (e = hipEventRecord(e0, h->stream)) != hipSuccess) break;
-            if ((rc = enqueue_step(h, dt, false))) return finish(rc);
+            if ((rc = enqueue_step(h, dt, false))) return abandon(rc);
             h->cur ^= 1;
         }
         if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
@@ -1281,10 +1294,8 @@ int sph_autotune(sph_handle h, float dt, uint32_t trial_steps, uint32_t* chosen)
         h->cur = c;
         if (e == hipSuccess) e = restore();
         if (e != hipSuccess) {
-            h->flags = base | best | handoff(best, bestLists);
-            h->cwList = bestLists;
             h->err = std::string("sph_autotune: ") + hipGetErrorString(e);
-            return finish(SPH_E_HIP);
+            return abandon(SPH_E_HIP);
         }
         if (first || ms < 0.97f * bestMs) {
             best = w;
