@@ -247,6 +247,16 @@ hipEvent_t take_event(sph_engine* h) {
     return e;
 }
 
+// The engine's device staging buffer for readbacks (9 N floats: positions,
+// velocities, pressures and keys in original order, then N slot ids), allocated
+// on the first readback after initialize and kept: the synchronous readers and
+// the asynchronous ring share it, all in stream order on the engine's stream.
+int ensure_staging(sph_engine* h) {
+    if (h->rbDev || h->N == 0) return SPH_OK;
+    SPH_HIP(h, hipMalloc((void**)&h->rbDev, sizeof(float) * 9 * h->N));
+    return SPH_OK;
+}
+
 // Folds finished stage events into the totals.
 int collect_events(sph_engine* h) {
     for (auto& pe : h->pending) {
@@ -856,22 +866,16 @@ int sph_read(sph_handle h, float* pos, float* vel, float* pressure) {
     if (h->N == 0) return SPH_OK;
     SPH_HIP(h, hipSetDevice(h->device));
     const uint64_t N = h->N;
-    float *dP = nullptr, *dV = nullptr, *dPr = nullptr;
-    uint32_t *dK = nullptr, *dI = nullptr;
-    int rc = SPH_OK;
-    if ((rc = dalloc(h, &dP, 3 * N)) || (rc = dalloc(h, &dV, 3 * N)) || (rc = dalloc(h, &dPr, N)) ||
-        (rc = dalloc(h, &dK, N)) || (rc = dalloc(h, &dI, N))) {
-        dfree(dP), dfree(dV), dfree(dPr), dfree(dK), dfree(dI);
-        return rc;
-    }
+    int rc = ensure_staging(h);
+    if (rc) return rc;
+    float* d = h->rbDev;
     hipLaunchKernelGGL(k_unsort, dim3((uint32_t)((N + 255) / 256)), dim3(256), 0, h->stream, h->pos, h->vel,
-                       h->ids[h->cur], h->origOfSlot, (uint32_t)N, dP, dV, dPr, dK, dI);
+                       h->ids[h->cur], h->origOfSlot, (uint32_t)N, d, d + 3 * N, d + 6 * N, nullptr, nullptr);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    if (e == hipSuccess && pos) e = hipMemcpy(pos, dP, sizeof(float) * 3 * N, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && vel) e = hipMemcpy(vel, dV, sizeof(float) * 3 * N, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && pressure) e = hipMemcpy(pressure, dPr, sizeof(float) * N, hipMemcpyDeviceToHost);
-    dfree(dP), dfree(dV), dfree(dPr), dfree(dK), dfree(dI);
+    if (e == hipSuccess && pos) e = hipMemcpy(pos, d, sizeof(float) * 3 * N, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && vel) e = hipMemcpy(vel, d + 3 * N, sizeof(float) * 3 * N, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && pressure) e = hipMemcpy(pressure, d + 6 * N, sizeof(float) * N, hipMemcpyDeviceToHost);
     if (e != hipSuccess) {
         h->err = std::string("sph_read: ") + hipGetErrorString(e);
         return SPH_E_HIP;
@@ -888,9 +892,8 @@ int sph_readback_enqueue(sph_handle h, uint32_t* slot) {
     SPH_HIP(h, hipSetDevice(h->device));
     const uint64_t N = h->N;
     const uint64_t words = 8 * N;  // pos 3N, vel 3N, pressure N, key N
-    if (!h->rbEvent[0] || h->rbN != N || (N && !h->rbDev)) {  // first use after initialize
-        dfree(h->rbDev);
-        int rc = dalloc(h, &h->rbDev, words + N);
+    if (!h->rbEvent[0] || h->rbN != N || (N && !h->rbHost[0])) {  // first use after initialize
+        int rc = ensure_staging(h);
         if (rc) return rc;
         for (uint32_t k = 0; k < sph_engine::kReadbackSlots; ++k) {
             if (h->rbHost[k]) (void)hipHostFree(h->rbHost[k]);
@@ -968,21 +971,17 @@ int read_keys_ids(sph_engine* h, uint32_t* keys, uint32_t* bucketIds) {
     if (h->N == 0) return SPH_OK;
     SPH_HIP(h, hipSetDevice(h->device));
     const uint64_t N = h->N;
-    float *dP = nullptr, *dV = nullptr, *dPr = nullptr;
-    uint32_t *dK = nullptr, *dI = nullptr;
-    int rc = SPH_OK;
-    if ((rc = dalloc(h, &dP, 3 * N)) || (rc = dalloc(h, &dV, 3 * N)) || (rc = dalloc(h, &dPr, N)) ||
-        (rc = dalloc(h, &dK, N)) || (rc = dalloc(h, &dI, N))) {
-        dfree(dP), dfree(dV), dfree(dPr), dfree(dK), dfree(dI);
-        return rc;
-    }
+    int rc = ensure_staging(h);
+    if (rc) return rc;
+    float* d = h->rbDev;
+    uint32_t* dK = reinterpret_cast<uint32_t*>(d + 7 * N);
+    uint32_t* dI = reinterpret_cast<uint32_t*>(d + 8 * N);
     hipLaunchKernelGGL(k_unsort, dim3((uint32_t)((N + 255) / 256)), dim3(256), 0, h->stream, h->pos, h->vel,
-                       h->ids[h->cur], h->origOfSlot, (uint32_t)N, dP, dV, dPr, dK, dI);
+                       h->ids[h->cur], h->origOfSlot, (uint32_t)N, d, d + 3 * N, d + 6 * N, dK, dI);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e == hipSuccess && keys) e = hipMemcpy(keys, dK, sizeof(uint32_t) * N, hipMemcpyDeviceToHost);
     if (e == hipSuccess && bucketIds) e = hipMemcpy(bucketIds, dI, sizeof(uint32_t) * N, hipMemcpyDeviceToHost);
-    dfree(dP), dfree(dV), dfree(dPr), dfree(dK), dfree(dI);
     if (e != hipSuccess) {
         h->err = std::string("read: ") + hipGetErrorString(e);
         return SPH_E_HIP;
@@ -1044,12 +1043,9 @@ int sph_read_debug(sph_handle h, float* density, float* accel) {
     }
     SPH_HIP(h, hipSetDevice(h->device));
     const uint64_t N = h->N;
-    float *dD = nullptr, *dA = nullptr;
-    int rc;
-    if ((rc = dalloc(h, &dD, N)) || (rc = dalloc(h, &dA, 3 * N))) {
-        dfree(dD), dfree(dA);
-        return rc;
-    }
+    int rc = ensure_staging(h);
+    if (rc) return rc;
+    float *dD = h->rbDev, *dA = h->rbDev + N;
     // The debug arrays are in the order of the last step's input: ids[cur ^ 1].
     hipLaunchKernelGGL(k_unsort_debug, dim3((uint32_t)((N + 255) / 256)), dim3(256), 0, h->stream, h->dens, h->acc,
                        h->ids[h->cur ^ 1], h->origOfSlot, (uint32_t)N, dD, dA);
@@ -1057,7 +1053,6 @@ int sph_read_debug(sph_handle h, float* density, float* accel) {
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e == hipSuccess && density) e = hipMemcpy(density, dD, sizeof(float) * N, hipMemcpyDeviceToHost);
     if (e == hipSuccess && accel) e = hipMemcpy(accel, dA, sizeof(float) * 3 * N, hipMemcpyDeviceToHost);
-    dfree(dD), dfree(dA);
     if (e != hipSuccess) {
         h->err = std::string("sph_read_debug: ") + hipGetErrorString(e);
         return SPH_E_HIP;

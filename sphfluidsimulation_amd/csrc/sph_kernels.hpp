@@ -973,8 +973,8 @@ __global__ __launch_bounds__(kBlock) void k_unsort(const float4* __restrict__ po
     outVel[3ull * o + 1] = v.y;
     outVel[3ull * o + 2] = v.z;
     outP[o] = p.w;
-    outKey[o] = __float_as_uint(v.w);
-    outIds[k] = o;
+    if (outKey) outKey[o] = __float_as_uint(v.w);
+    if (outIds) outIds[k] = o;
 }
 
 // Per-particle debug arrays of the last step (indexed in that step's order).
