@@ -138,6 +138,17 @@ int sph_set_triangles(sph_handle h, uint32_t scene, const float* tri9, uint64_t 
  * (SimulationEngineCPU.cpp:422-429). */
 int sph_update(sph_handle h, float dt, uint32_t steps);
 
+/* No counterpart in the reference: sph_read for callers that cannot allocate
+ * device memory (serving loops and harnesses that forbid hipMalloc after set-up,
+ * code next to HIP graph capture). Same results as sph_read; its staging is a
+ * fixed device array of SPH_NOALLOC_MAX_PARTICLES particles (28 bytes each,
+ * 28 MiB) that each GPU holds from sph_create until the process ends, so the call
+ * allocates nothing. Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (two threads, two handles) race. An engine with more
+ * particles returns SPH_E_INVALID. */
+#define SPH_NOALLOC_MAX_PARTICLES (1u << 20)
+int sph_read_noalloc(sph_handle h, float* pos, float* vel, float* pressure);
+
 /* Asynchronous readback, the HIP form of the reference's renderable buffer
  * manager (RenderableGPUParticleBufferManager.cpp:190-270: a ring of buffers, a
  * read lock that waits on an event, OpenCLResourceLock.cpp:73-163). The engine

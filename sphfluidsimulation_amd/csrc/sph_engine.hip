@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -21,6 +22,13 @@
 #include "sph_cellwave.hpp"
 
 using namespace sph;
+
+// Staging of sph_read_noalloc: positions (3 N), velocities (3 N) and pressures
+// (N) in original order, N <= SPH_NOALLOC_MAX_PARTICLES. Each GPU holds its own
+// copy from the module's load until the process ends. Host code never uses this
+// name as an address: sph_create resolves each device's copy with
+// hipGetSymbolAddress (noalloc_scratch) and the engine passes that pointer.
+static __device__ __align__(256) float scratch[7ull * SPH_NOALLOC_MAX_PARTICLES];
 
 namespace {
 
@@ -34,6 +42,7 @@ struct StageEvent {
 struct sph_engine {
     int device = 0;
     hipStream_t stream = nullptr;
+    float* scratch = nullptr;  // this device's copy of the file-scope `scratch` array
     std::string err;
     bool initialized = false;
     uint32_t flags = 0;
@@ -484,6 +493,27 @@ int capture_graph(sph_engine* h, int parity, float dt) {
 
 bool valid(sph_handle h) { return h != nullptr; }
 
+// Device address of each GPU's copy of `scratch`, resolved once per device with
+// that device current.
+constexpr int kMaxDevices = 64;
+std::mutex gScratchMutex;
+float* gScratch[kMaxDevices] = {};
+
+int noalloc_scratch(sph_engine* h) {
+    if (h->device >= kMaxDevices) {
+        h->err = "device ordinal above the library's table of per-device arrays";
+        return SPH_E_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(gScratchMutex);
+    if (!gScratch[h->device]) {
+        void* p = nullptr;
+        SPH_HIP(h, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        gScratch[h->device] = static_cast<float*>(p);
+    }
+    h->scratch = gScratch[h->device];
+    return SPH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -499,8 +529,15 @@ int sph_create(sph_handle* out, int device) {
     sph_engine* h = new sph_engine;
     h->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        if (h->stream) (void)hipStreamDestroy(h->stream);
         delete h;
         return SPH_E_HIP;
+    }
+    const int rc = noalloc_scratch(h);
+    if (rc) {
+        (void)hipStreamDestroy(h->stream);
+        delete h;
+        return rc;
     }
     *out = h;
     return SPH_OK;
@@ -878,6 +915,36 @@ int sph_read(sph_handle h, float* pos, float* vel, float* pressure) {
     if (e == hipSuccess && pressure) e = hipMemcpy(pressure, d + 6 * N, sizeof(float) * N, hipMemcpyDeviceToHost);
     if (e != hipSuccess) {
         h->err = std::string("sph_read: ") + hipGetErrorString(e);
+        return SPH_E_HIP;
+    }
+    return SPH_OK;
+}
+
+int sph_read_noalloc(sph_handle h, float* pos, float* vel, float* pressure) {
+    if (!valid(h)) return SPH_E_INVALID;
+    if (!h->initialized) {
+        h->err = "sph_read_noalloc: engine not initialized";
+        return SPH_E_STATE;
+    }
+    if (h->N > SPH_NOALLOC_MAX_PARTICLES) {
+        h->err = "sph_read_noalloc: " + std::to_string(h->N) + " particles, more than SPH_NOALLOC_MAX_PARTICLES (" +
+                 std::to_string(SPH_NOALLOC_MAX_PARTICLES) + ")";
+        return SPH_E_INVALID;
+    }
+    if (h->N == 0) return SPH_OK;
+    SPH_HIP(h, hipSetDevice(h->device));
+    // k_unsort writes every entry of [0, 7 N): nothing needs clearing first.
+    const uint64_t N = h->N;
+    float* d = h->scratch;
+    hipLaunchKernelGGL(k_unsort, dim3((uint32_t)((N + 255) / 256)), dim3(256), 0, h->stream, h->pos, h->vel,
+                       h->ids[h->cur], h->origOfSlot, (uint32_t)N, d, d + 3 * N, d + 6 * N, nullptr, nullptr);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess && pos) e = hipMemcpy(pos, d, sizeof(float) * 3 * N, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && vel) e = hipMemcpy(vel, d + 3 * N, sizeof(float) * 3 * N, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && pressure) e = hipMemcpy(pressure, d + 6 * N, sizeof(float) * N, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        h->err = std::string("sph_read_noalloc: ") + hipGetErrorString(e);
         return SPH_E_HIP;
     }
     return SPH_OK;

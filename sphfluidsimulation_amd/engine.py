@@ -134,6 +134,17 @@ class SimulationEngineHIP:
                                        _ptr(p, ctypes.c_float)))
         return pos, vel, p
 
+    def read_noalloc(self):
+        """read() through sph_read_noalloc: staging in the library's fixed device
+        array (at most SPH_NOALLOC_MAX_PARTICLES particles), no device allocation."""
+        n = self.n
+        pos = np.empty((n, 3), np.float32)
+        vel = np.empty((n, 3), np.float32)
+        p = np.empty(n, np.float32)
+        self._check(self._lib.sph_read_noalloc(self._h, _ptr(pos, ctypes.c_float), _ptr(vel, ctypes.c_float),
+                                               _ptr(p, ctypes.c_float)))
+        return pos, vel, p
+
     def read_keys(self):
         k = np.empty(self.n, np.uint32)
         self._check(self._lib.sph_read_keys(self._h, _ptr(k, ctypes.c_uint32)))
