@@ -57,6 +57,13 @@ extern "C" {
 #define SPH_FLAG_CW_LISTS 0x40u     /* default walk: neighbour lists whatever the particle count */
 #define SPH_FLAG_CW_MASKS 0x80u     /* default walk: neighbour masks whatever the particle count */
 #define SPH_CW_LIST_MAX 1572864u    /* particles up to which the default walk uses lists */
+#define SPH_FLAG_PARTICLE_MAP 0x100u /* the reference's particle-map mode (SimulationEngineCPU.cpp:280-316):
+                                       the state keeps its memory order between reorders and the buckets
+                                       list it through a particle map refilled every step (SPHFunctions.cpp:
+                                       346-356); a step reorders the state when the reorder interval has
+                                       elapsed (sph_set_reorder_interval). Walks cell by cell, as the
+                                       reference's loop does. Default (no flag): the state is reordered
+                                       every step, the fastest choice on the GPU */
 
 /* sph_validate result bits (the invariants of SimulationEngine.h:115-282). */
 #define SPH_V_NONFINITE 0x1u        /* NaN or Inf in position, velocity or pressure */
@@ -131,12 +138,33 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
  * SimulationEngineCPU.cpp:418): count triangles of 9 floats (p1, p2, p3). */
 int sph_set_triangles(sph_handle h, uint32_t scene, const float* tri9, uint64_t count);
 
+/* The scene's reorderTimeInterval (SimulationEngineCPU.cpp:355, 400; read from the
+ * scene's otherParameters): in SPH_FLAG_PARTICLE_MAP mode, every step of an update
+ * whose start finds the time since the last reorder above the interval reorders
+ * the state; the time advances by dt once per update and restarts at 0 in the
+ * update that takes it above the interval (SimulationEngineCPU.cpp:446, 455-457),
+ * so the next update's check finds 0: as in the reference, a positive interval
+ * never reorders and a negative one reorders every update. Initialize resets it
+ * to FLT_MAX (SimulationEngineCPU.cpp:389-390). No effect without the flag. */
+int sph_set_reorder_interval(sph_handle h, float interval);
+
 /* SimulationEngine::Update (SimulationEngine.h:42; SimulationEngineCPU.cpp:420-459):
  * enqueues `steps` steps of size dt on the engine's stream and returns. Unlike
  * the reference's CPU engine it never drops an update. Update before Initialize
  * returns SPH_E_STATE; n == 0 returns SPH_OK and does nothing
  * (SimulationEngineCPU.cpp:422-429). */
 int sph_update(sph_handle h, float dt, uint32_t steps);
+
+/* ParticleBufferManager::LockWrite of the current buffer (ParticleBufferManager.h:22,
+ * OfflineCPUParticleBufferManager.cpp:68-74) followed by its unlock: replaces the
+ * positions and/or velocities (3 floats per particle, original order; either may
+ * be NULL to keep it) of the state the next Update starts from. As in the
+ * reference, whose engines walk a written buffer through the tables of their last
+ * step (SimulationEngineCPU.cpp:132-349), the stored keys, the bucket tables and
+ * the pressures stay as they are until that Update rehashes; sph_validate reports
+ * SPH_V_KEY for a written particle that left its cell. Synchronous: waits for
+ * the steps already enqueued. */
+int sph_write(sph_handle h, const float* pos, const float* vel);
 
 /* No counterpart in the reference: sph_read for callers that cannot allocate
  * device memory (serving loops and harnesses that forbid hipMalloc after set-up,

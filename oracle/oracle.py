@@ -34,6 +34,7 @@ _SIGS = {
     "sph_oracle_hash_size": (ctypes.c_uint64, [_P]),
     "sph_oracle_simulation_time": (ctypes.c_double, [_P]),
     "sph_oracle_read": (ctypes.c_int, [_P, _F64P, _F64P, _F64P]),
+    "sph_oracle_write": (ctypes.c_int, [_P, _F32P, _F32P]),
     "sph_oracle_read_keys": (ctypes.c_int, [_P, _U32P]),
     "sph_oracle_read_hash_map": (ctypes.c_int, [_P, _U32P]),
     "sph_oracle_read_bucket_ids": (ctypes.c_int, [_P, _U32P]),
@@ -129,6 +130,14 @@ class Oracle:
         self._lib.sph_oracle_read(self._h, _p(pos, ctypes.c_double), _p(vel, ctypes.c_double),
                                   _p(p, ctypes.c_double))
         return pos, vel, p
+
+    def write(self, pos=None, vel=None):
+        """LockWrite of the current buffer: positions and/or velocities by original
+        index; hash, pressure and tables unchanged until the next update."""
+        p = None if pos is None else _f32(pos).reshape(self.n, 3)
+        v = None if vel is None else _f32(vel).reshape(self.n, 3)
+        self._keep_write = (p, v)
+        assert self._lib.sph_oracle_write(self._h, _p(p, ctypes.c_float), _p(v, ctypes.c_float)) == 0
 
     def read_keys(self):
         k = np.empty(self.n, np.uint32)

@@ -760,6 +760,26 @@ int sph_oracle_read(void* h, double* pos, double* vel, double* pressure) {
     return H->fp64 ? rd(*H->d) : rd(*H->f);
 }
 
+// OfflineCPUParticleBufferManager::LockWrite of the current buffer
+// (OfflineCPUParticleBufferManager.cpp:68-74) and a caller writing positions
+// and/or velocities (original order; either may be null) into its records: the
+// stored hash, the pressure, the hash map and particleMap stay as they are, so
+// the next update walks the buckets of the last step (CPU:132-349).
+int sph_oracle_write(void* h, const float* pos, const float* vel) {
+    Handle* H = (Handle*)h;
+    auto wr = [&](auto& E) {
+        using R = decltype(E.P.h);
+        auto* P = E.buf[E.cur].data();
+        for (uint64_t s = 0; s < E.n; ++s) {
+            const uint32_t o = E.origOfSlot[s];
+            if (pos) P[s].pos = {(R)pos[3 * o], (R)pos[3 * o + 1], (R)pos[3 * o + 2]};
+            if (vel) P[s].vel = {(R)vel[3 * o], (R)vel[3 * o + 1], (R)vel[3 * o + 2]};
+        }
+        return 0;
+    };
+    return H->fp64 ? wr(*H->d) : wr(*H->f);
+}
+
 // Stored bucket key (velocityAndHash.w) by original particle index.
 int sph_oracle_read_keys(void* h, uint32_t* keys) {
     Handle* H = (Handle*)h;

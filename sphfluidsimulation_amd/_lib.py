@@ -30,6 +30,7 @@ SPH_FLAG_TILE_WALK = 0x20
 SPH_FLAG_CW_LISTS = 0x40
 SPH_FLAG_CW_MASKS = 0x80
 SPH_CW_LIST_MAX = 1572864
+SPH_FLAG_PARTICLE_MAP = 0x100
 SPH_NOALLOC_MAX_PARTICLES = 1 << 20
 
 SPH_V_NONFINITE = 0x1
@@ -76,6 +77,8 @@ SIGNATURES = {
     "sph_synchronize": (ctypes.c_int, [_P]),
     "sph_read": (ctypes.c_int, [_P, _F32P, _F32P, _F32P]),
     "sph_read_noalloc": (ctypes.c_int, [_P, _F32P, _F32P, _F32P]),
+    "sph_write": (ctypes.c_int, [_P, _F32P, _F32P]),
+    "sph_set_reorder_interval": (ctypes.c_int, [_P, ctypes.c_float]),
     "sph_read_keys": (ctypes.c_int, [_P, _U32P]),
     "sph_read_cell_starts": (ctypes.c_int, [_P, _U32P]),
     "sph_read_bucket_ids": (ctypes.c_int, [_P, _U32P]),

@@ -64,6 +64,31 @@ def build_adapter_demo():
     return ADAPTER_BIN
 
 
+REFERENCE_DIR = os.path.join(PKG, "reference")
+REFERENCE_SRCS = [
+    os.path.join(REFERENCE_DIR, "source", "SPH", "SimulationEngines", "SimulationEngineHIP.cpp"),
+    os.path.join(REFERENCE_DIR, "source", "SPH", "ParticleBufferManagers", "HIPParticleBufferManager.cpp"),
+]
+MIRROR_DIR = os.path.join(ROOT, "tests", "cpp", "refmirror")
+BINDING_SRC = os.path.join(ROOT, "tests", "cpp", "reference_binding.cpp")
+BINDING_BIN = os.path.join(ROOT, "tests", "cpp", "reference_binding")
+
+
+def reference_binding_cmd(extra=()):
+    """g++ line for the drop-in reference engine (sphfluidsimulation_amd/reference/)
+    against the declaration mirror of the reference's interface (tests/cpp/refmirror/)."""
+    return ["g++", "-std=c++20", "-O2", "-Wall", "-Wextra", "-Wno-unused-parameter", *extra,
+            "-I" + MIRROR_DIR, "-I" + os.path.join(REFERENCE_DIR, "include"), "-I" + os.path.join(ROOT, "include"),
+            *REFERENCE_SRCS, os.path.join(MIRROR_DIR, "refmirror.cpp"), BINDING_SRC]
+
+
+def build_reference_binding():
+    """The drop-in SimulationEngineHIP driven through the reference's interface
+    (a host-only g++ build against libsph_hip.so)."""
+    _run(reference_binding_cmd() + ["-o", BINDING_BIN, LIB, "-Wl,-rpath," + PKG])
+    return BINDING_BIN
+
+
 WALK_SRC = os.path.join(ROOT, "tests", "cpp", "walk_check.cpp")
 WALK_BIN = os.path.join(ROOT, "tests", "cpp", "walk_check")
 
@@ -78,6 +103,7 @@ def build_all():
     build_hip()
     build_oracle()
     build_adapter_demo()
+    build_reference_binding()
     build_walk_check()
 
 

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -93,6 +94,13 @@ struct sph_engine {
     double* consOut = nullptr;
     uint32_t* valBits = nullptr;
     uint32_t* valIdx = nullptr;  // 4 * nScenes: firstIdx, firstVal, lastIdx, lastVal (dynamic), then static
+    // Particle-map mode (SPH_FLAG_PARTICLE_MAP): bucket slot k holds particle pmap[k]
+    // while the state keeps its memory order; a reordering step sorts the state and
+    // resets the map (SimulationEngineCPU.cpp:280-316, 420-459).
+    uint32_t* pmap = nullptr;
+    float reorderInterval = FLT_MAX;  // reorderTimeInterval (SimulationEngineCPU.cpp:355)
+    float reorderElapsed = 0.f;
+    bool reorderNow = false;          // the steps being enqueued reorder
     int cur = 0;                 // ids[cur] / tab[cur] hold the current sorted state
     bool debugValid = false;
     double simTime = 0.0;
@@ -109,9 +117,10 @@ struct sph_engine {
     double rbTime[kReadbackSlots] = {0, 0, 0};
     uint32_t rbNext = 0;
 
-    // hipGraph replay: one graph per starting parity, keyed by dt.
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
-    float graphDt[2] = {0.f, 0.f};
+    // hipGraph replay: one graph per starting parity and (particle-map mode)
+    // reordering or not, index 2 * reorderNow + parity, keyed by dt.
+    hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
+    float graphDt[4] = {0.f, 0.f, 0.f, 0.f};
 
     // Profiling.
     uint32_t profiling = 0;  // stage mask
@@ -196,6 +205,10 @@ void release(sph_engine* h) {
     dfree(h->consOut);
     dfree(h->valBits);
     dfree(h->valIdx);
+    dfree(h->pmap);
+    h->reorderInterval = FLT_MAX;  // SimulationEngineCPU.cpp:389-390
+    h->reorderElapsed = 0.f;
+    h->reorderNow = false;
     dfree(h->rbDev);
     for (uint32_t k = 0; k < sph_engine::kReadbackSlots; ++k) {
         if (h->rbHost[k]) (void)hipHostFree(h->rbHost[k]);
@@ -285,7 +298,7 @@ int collect_events(sph_engine* h) {
 enum class Walk { CellWave, Runs, Tiles, Cells };
 
 Walk walk_of(uint32_t flags) {
-    if (flags & SPH_FLAG_CELL_WALK) return Walk::Cells;
+    if (flags & (SPH_FLAG_CELL_WALK | SPH_FLAG_PARTICLE_MAP)) return Walk::Cells;
     if (flags & SPH_FLAG_RUN_WALK) return Walk::Runs;
     if (flags & SPH_FLAG_TILE_WALK) return Walk::Tiles;
     return Walk::CellWave;
@@ -363,20 +376,25 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
         }
 #undef SPH_DENS
 #undef SPH_DENS_CW
-    } else if (h->hasStatic) {
-        if (dbg)
-            hipLaunchKernelGGL((k_density<true, true>), grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->pos,
-                               h->tab[c], h->spos, h->stab, h->pres, h->dens);
-        else
-            hipLaunchKernelGGL((k_density<true, false>), grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->pos,
-                               h->tab[c], h->spos, h->stab, h->pres, h->dens);
     } else {
-        if (dbg)
-            hipLaunchKernelGGL((k_density<false, true>), grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->pos,
-                               h->tab[c], h->spos, h->stab, h->pres, h->dens);
-        else
-            hipLaunchKernelGGL((k_density<false, false>), grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->pos,
-                               h->tab[c], h->spos, h->stab, h->pres, h->dens);
+#define SPH_DENS_CELL(S, D, P)                                                                                   \
+    hipLaunchKernelGGL((k_density<S, D, P>), grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->pos, h->tab[c], \
+                       h->spos, h->stab, h->pmap, h->pres, h->dens)
+        const bool map = h->pmap != nullptr;
+        if (h->hasStatic) {
+            if (dbg) {
+                if (map) SPH_DENS_CELL(true, true, true); else SPH_DENS_CELL(true, true, false);
+            } else {
+                if (map) SPH_DENS_CELL(true, false, true); else SPH_DENS_CELL(true, false, false);
+            }
+        } else {
+            if (dbg) {
+                if (map) SPH_DENS_CELL(false, true, true); else SPH_DENS_CELL(false, true, false);
+            } else {
+                if (map) SPH_DENS_CELL(false, false, true); else SPH_DENS_CELL(false, false, false);
+            }
+        }
+#undef SPH_DENS_CELL
     }
     SPH_HIP(h, hipGetLastError());
     if ((rc = end())) return rc;
@@ -411,10 +429,14 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
         hipLaunchKernelGGL((k_force_runs<S, D, T>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,        \
                            h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->mask, h->ids[c], h->tris,  \
                            dt, h->posT, h->velT, h->tab[c ^ 1], h->acc);                                          \
+    else if (h->pmap)                                                                                             \
+        hipLaunchKernelGGL((k_force<S, D, T, true>), grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->pos,   \
+                           h->vel, h->pres, h->tab[c], h->spos, h->stab, h->pmap, h->ids[c], h->tris, dt, h->posT, \
+                           h->velT, h->tab[c ^ 1], h->acc);                                                       \
     else                                                                                                          \
-        hipLaunchKernelGGL((k_force<S, D, T>), grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->pos, h->vel, \
-                           h->pres, h->tab[c], h->spos, h->stab, h->ids[c], h->tris, dt, h->posT, h->velT,        \
-                           h->tab[c ^ 1], h->acc);                                                                \
+        hipLaunchKernelGGL((k_force<S, D, T, false>), grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->pos,  \
+                           h->vel, h->pres, h->tab[c], h->spos, h->stab, nullptr, h->ids[c], h->tris, dt, h->posT,\
+                           h->velT, h->tab[c ^ 1], h->acc);                                                       \
     } while (0)
     if (h->hasStatic) {
         if (dbg) {
@@ -451,9 +473,13 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
         hipLaunchKernelGGL(k_scatter_order, dim3(h->numBlocks + 8u), block, 0, h->stream, h->dBlocks, h->numBlocks,
                            h->dScenes, h->posT, h->velT, h->ids[c], h->tab[c ^ 1], h->pos, h->vel, h->ids[c ^ 1],
                            h->heavy, h->tileOrder);
+    } else if (h->pmap && !h->reorderNow) {
+        // Particle-map mode between reorders: memory order kept, map refilled.
+        hipLaunchKernelGGL(k_fill_map, grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->posT, h->velT,
+                           h->ids[c], h->tab[c ^ 1], h->pos, h->vel, h->ids[c ^ 1], h->pmap);
     } else {
         hipLaunchKernelGGL(k_scatter, grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->posT, h->velT,
-                           h->ids[c], h->tab[c ^ 1], h->pos, h->vel, h->ids[c ^ 1]);
+                           h->ids[c], h->tab[c ^ 1], h->pos, h->vel, h->ids[c ^ 1], h->pmap);
     }
     SPH_HIP(h, hipGetLastError());
     if ((rc = end())) return rc;
@@ -461,9 +487,10 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
 }
 
 int capture_graph(sph_engine* h, int parity, float dt) {
-    if (h->graph[parity]) {
-        SPH_HIP(h, hipGraphExecDestroy(h->graph[parity]));
-        h->graph[parity] = nullptr;
+    const int gi = 2 * (h->reorderNow ? 1 : 0) + parity;
+    if (h->graph[gi]) {
+        SPH_HIP(h, hipGraphExecDestroy(h->graph[gi]));
+        h->graph[gi] = nullptr;
     }
     const int saved = h->cur;
     h->cur = parity;
@@ -480,14 +507,14 @@ int capture_graph(sph_engine* h, int parity, float dt) {
         h->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
         return SPH_E_HIP;
     }
-    e = hipGraphInstantiate(&h->graph[parity], g, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(&h->graph[gi], g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) {
         h->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
-        h->graph[parity] = nullptr;
+        h->graph[gi] = nullptr;
         return SPH_E_HIP;
     }
-    h->graphDt[parity] = dt;
+    h->graphDt[gi] = dt;
     return SPH_OK;
 }
 
@@ -746,7 +773,8 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
     SPH_ALLOC(h->valIdx, 8ull * n_scenes);
     static_assert(kMaskWords == kTileMaskWords && kMaskWords == kCwMaskWords,
                   "every mask format uses 8 words per particle");
-    if (!(flags & SPH_FLAG_CELL_WALK)) {
+    if (flags & SPH_FLAG_PARTICLE_MAP) SPH_ALLOC(h->pmap, std::max<uint64_t>(N, 1));
+    if (walk_of(flags) != Walk::Cells) {
         SPH_ALLOC(h->mask, kMaskWords * std::max<uint64_t>(N, 1));
         if (kViscLists) SPH_ALLOC(h->visc, std::max<uint64_t>(N, 1));
         // Lists while they stay in the caches (4 B per neighbour), else masks (32 B per particle).
@@ -772,6 +800,9 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
         SPH_HIP(h, hipMemcpy(h->pos, hPos.data(), sizeof(float4) * N, hipMemcpyHostToDevice));
         SPH_HIP(h, hipMemcpy(h->vel, hVel.data(), sizeof(float4) * N, hipMemcpyHostToDevice));
         SPH_HIP(h, hipMemcpy(h->ids[0], hIds.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice));
+        // The initial sort moved the state into bucket order (SimulationEngineCPU.cpp:55-131):
+        // the particle map starts as the identity (hIds holds slot k at k).
+        if (h->pmap) SPH_HIP(h, hipMemcpy(h->pmap, hIds.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice));
         SPH_HIP(h, hipMemcpy(h->origOfSlot, hOrig.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice));
         SPH_HIP(h, hipMemset(h->pres, 0, sizeof(float) * N));
     }
@@ -859,6 +890,16 @@ int sph_set_triangles(sph_handle h, uint32_t scene, const float* tri9, uint64_t 
     return SPH_OK;
 }
 
+int sph_set_reorder_interval(sph_handle h, float interval) {
+    if (!valid(h)) return SPH_E_INVALID;
+    if (!h->initialized) {
+        h->err = "sph_set_reorder_interval: engine not initialized";
+        return SPH_E_STATE;
+    }
+    h->reorderInterval = interval;
+    return SPH_OK;
+}
+
 int sph_update(sph_handle h, float dt, uint32_t steps) {
     if (!valid(h)) return SPH_E_INVALID;
     if (!h->initialized) {
@@ -868,14 +909,17 @@ int sph_update(sph_handle h, float dt, uint32_t steps) {
     if (h->N == 0 || steps == 0) return SPH_OK;  // SimulationEngineCPU.cpp:428-429
     SPH_HIP(h, hipSetDevice(h->device));
     const bool useGraph = (h->flags & SPH_FLAG_HIP_GRAPH) && !h->profiling;
+    // Particle-map mode: every step of this update reorders when the interval has
+    // elapsed (SimulationEngineCPU.cpp:446), checked once per update.
+    h->reorderNow = h->pmap && h->reorderElapsed > h->reorderInterval;
     for (uint32_t s = 0; s < steps; ++s) {
         if (useGraph) {
-            const int c = h->cur;
-            if (!h->graph[c] || h->graphDt[c] != dt) {
+            const int c = h->cur, gi = 2 * (h->reorderNow ? 1 : 0) + c;
+            if (!h->graph[gi] || h->graphDt[gi] != dt) {
                 int rc = capture_graph(h, c, dt);
                 if (rc) return rc;
             }
-            SPH_HIP(h, hipGraphLaunch(h->graph[c], h->stream));
+            SPH_HIP(h, hipGraphLaunch(h->graph[gi], h->stream));
         } else {
             int rc = enqueue_step(h, dt, false);
             if (rc) return rc;
@@ -884,6 +928,11 @@ int sph_update(sph_handle h, float dt, uint32_t steps) {
     }
     h->debugValid = (h->flags & SPH_FLAG_RECORD_DEBUG) != 0;
     h->simTime += (double)dt * steps;  // SimulationEngineCPU.cpp:453
+    if (h->pmap) {  // SimulationEngineCPU.cpp:455-457: once per update
+        h->reorderElapsed += dt;
+        if (h->reorderElapsed > h->reorderInterval) h->reorderElapsed = 0.f;
+    }
+    h->reorderNow = false;
     return SPH_OK;
 }
 
@@ -907,7 +956,7 @@ int sph_read(sph_handle h, float* pos, float* vel, float* pressure) {
     if (rc) return rc;
     float* d = h->rbDev;
     hipLaunchKernelGGL(k_unsort, dim3((uint32_t)((N + 255) / 256)), dim3(256), 0, h->stream, h->pos, h->vel,
-                       h->ids[h->cur], h->origOfSlot, (uint32_t)N, d, d + 3 * N, d + 6 * N, nullptr, nullptr);
+                       h->ids[h->cur], h->origOfSlot, (uint32_t)N, d, d + 3 * N, d + 6 * N, nullptr, nullptr, h->pmap);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e == hipSuccess && pos) e = hipMemcpy(pos, d, sizeof(float) * 3 * N, hipMemcpyDeviceToHost);
@@ -917,6 +966,29 @@ int sph_read(sph_handle h, float* pos, float* vel, float* pressure) {
         h->err = std::string("sph_read: ") + hipGetErrorString(e);
         return SPH_E_HIP;
     }
+    return SPH_OK;
+}
+
+int sph_write(sph_handle h, const float* pos, const float* vel) {
+    if (!valid(h)) return SPH_E_INVALID;
+    if (!h->initialized) {
+        h->err = "sph_write: engine not initialized";
+        return SPH_E_STATE;
+    }
+    if (h->N == 0 || (!pos && !vel)) return SPH_OK;
+    SPH_HIP(h, hipSetDevice(h->device));
+    const uint64_t N = h->N;
+    int rc = ensure_staging(h);
+    if (rc) return rc;
+    // The staging may still feed an asynchronous readback: wait for the stream first.
+    SPH_HIP(h, hipStreamSynchronize(h->stream));
+    float* d = h->rbDev;
+    if (pos) SPH_HIP(h, hipMemcpy(d, pos, sizeof(float) * 3 * N, hipMemcpyHostToDevice));
+    if (vel) SPH_HIP(h, hipMemcpy(d + 3 * N, vel, sizeof(float) * 3 * N, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_write, dim3((uint32_t)((N + 255) / 256)), dim3(256), 0, h->stream, h->pos, h->vel,
+                       h->ids[h->cur], h->origOfSlot, (uint32_t)N, pos ? d : nullptr, vel ? d + 3 * N : nullptr);
+    SPH_HIP(h, hipGetLastError());
+    SPH_HIP(h, hipStreamSynchronize(h->stream));
     return SPH_OK;
 }
 
@@ -937,7 +1009,7 @@ int sph_read_noalloc(sph_handle h, float* pos, float* vel, float* pressure) {
     const uint64_t N = h->N;
     float* d = h->scratch;
     hipLaunchKernelGGL(k_unsort, dim3((uint32_t)((N + 255) / 256)), dim3(256), 0, h->stream, h->pos, h->vel,
-                       h->ids[h->cur], h->origOfSlot, (uint32_t)N, d, d + 3 * N, d + 6 * N, nullptr, nullptr);
+                       h->ids[h->cur], h->origOfSlot, (uint32_t)N, d, d + 3 * N, d + 6 * N, nullptr, nullptr, h->pmap);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e == hipSuccess && pos) e = hipMemcpy(pos, d, sizeof(float) * 3 * N, hipMemcpyDeviceToHost);
@@ -977,7 +1049,7 @@ int sph_readback_enqueue(sph_handle h, uint32_t* slot) {
         float* d = h->rbDev;
         hipLaunchKernelGGL(k_unsort, dim3((uint32_t)((N + 255) / 256)), dim3(256), 0, h->stream, h->pos, h->vel,
                            h->ids[h->cur], h->origOfSlot, (uint32_t)N, d, d + 3 * N, d + 6 * N,
-                           reinterpret_cast<uint32_t*>(d + 7 * N), reinterpret_cast<uint32_t*>(d + 8 * N));
+                           reinterpret_cast<uint32_t*>(d + 7 * N), reinterpret_cast<uint32_t*>(d + 8 * N), h->pmap);
         SPH_HIP(h, hipGetLastError());
         SPH_HIP(h, hipMemcpyAsync(h->rbHost[k], d, sizeof(float) * words, hipMemcpyDeviceToHost, h->stream));
     }
@@ -1044,7 +1116,7 @@ int read_keys_ids(sph_engine* h, uint32_t* keys, uint32_t* bucketIds) {
     uint32_t* dK = reinterpret_cast<uint32_t*>(d + 7 * N);
     uint32_t* dI = reinterpret_cast<uint32_t*>(d + 8 * N);
     hipLaunchKernelGGL(k_unsort, dim3((uint32_t)((N + 255) / 256)), dim3(256), 0, h->stream, h->pos, h->vel,
-                       h->ids[h->cur], h->origOfSlot, (uint32_t)N, d, d + 3 * N, d + 6 * N, dK, dI);
+                       h->ids[h->cur], h->origOfSlot, (uint32_t)N, d, d + 3 * N, d + 6 * N, dK, dI, h->pmap);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e == hipSuccess && keys) e = hipMemcpy(keys, dK, sizeof(uint32_t) * N, hipMemcpyDeviceToHost);
@@ -1156,7 +1228,7 @@ int sph_validate(sph_handle h, uint32_t* bits) {
     const uint32_t ns = h->nScenes;
     if (h->numBlocks)
         hipLaunchKernelGGL(k_validate_particles, dim3(h->numBlocks), dim3(kBlock), 0, h->stream, h->dBlocks,
-                           h->dScenes, h->pos, h->vel, h->tab[h->cur], h->valBits);
+                           h->dScenes, h->pos, h->vel, h->tab[h->cur], h->pmap, h->valBits);
     {
         const uint64_t n = std::max<uint64_t>(h->L, ns);
         hipLaunchKernelGGL(k_validate_table, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, h->stream,
@@ -1250,7 +1322,9 @@ int sph_autotune(sph_handle h, float dt, uint32_t trial_steps, uint32_t* chosen)
         return walk == 0u ? (lists ? SPH_FLAG_CW_LISTS : SPH_FLAG_CW_MASKS) : 0u;
     };
     if (h->N == 0 || !h->mask) {  // nothing to time, or the engine was set up for the cell walk only
-        if (chosen) *chosen = (h->flags & walkBits) | handoff(h->flags & walkBits, h->cwList);
+        if (chosen)
+            *chosen = walk_of(h->flags) == Walk::Cells ? SPH_FLAG_CELL_WALK
+                                                       : (h->flags & walkBits) | handoff(h->flags & walkBits, h->cwList);
         return SPH_OK;
     }
     SPH_HIP(h, hipSetDevice(h->device));
@@ -1368,10 +1442,10 @@ int sph_autotune(sph_handle h, float dt, uint32_t trial_steps, uint32_t* chosen)
     }
     h->cwList = bestLists;
     h->flags = base | best | handoff(best, bestLists);
-    for (int p = 0; p < 2; ++p)  // captured steps may hold another walk's kernels
-        if (h->graph[p]) {
-            (void)hipGraphExecDestroy(h->graph[p]);
-            h->graph[p] = nullptr;
+    for (auto& g : h->graph)  // captured steps may hold another walk's kernels
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
         }
     h->debugValid = false;
     e = hipStreamSynchronize(h->stream);

@@ -160,16 +160,26 @@ constexpr uint32_t kMaskWords = kMaskBits / 32;
 
 // ---------------------------------------------------------------------------
 // k_density: SF:358-501 for one particle per lane.
-template <bool kStatic, bool kDebug>
+// kMap (particle-map mode): the state stays in memory order and bucket slot k of
+// the table holds particle pmap[k] (the reference's particleMap, SF:346-356);
+// lane k takes particle pmap[k], so a wave still walks neighbouring cells, and
+// each candidate slot j of a bucket is particle pmap[j]. Results go to the
+// particle's memory index. Without kMap, slot k is particle k.
+template <bool kStatic, bool kDebug, bool kMap>
 __global__ __launch_bounds__(kBlock) void k_density(const BlockDesc* __restrict__ blocks,
                                                     const SceneDev* __restrict__ scenes,
                                                     const float4* __restrict__ pos, const uint32_t* __restrict__ tab,
                                                     const float4* __restrict__ spos, const uint32_t* __restrict__ stab,
-                                                    float* __restrict__ pres, float* __restrict__ dens) {
+                                                    const uint32_t* __restrict__ pmap, float* __restrict__ pres,
+                                                    float* __restrict__ dens) {
     const BlockDesc bd = blocks[blockIdx.x];
     const SceneDev& S = scenes[bd.scene];
-    const uint32_t i = bd.begin + threadIdx.x;
-    if (i >= bd.end) return;
+    const uint32_t slot = bd.begin + threadIdx.x;
+    if (slot >= bd.end) return;
+    // The map is a permutation of the scene's particles; a corrupted entry is
+    // clamped so that no access leaves the arrays.
+    uint32_t i = kMap ? pmap[slot] : slot;
+    if (kMap && (i < S.pBegin || i >= S.pBegin + S.pCount)) i = S.pBegin;
     const float4 xi = pos[i];
     const float h = S.h, h2 = S.h2;
     const uint32_t pEnd = S.pBegin + S.pCount;
@@ -183,7 +193,8 @@ __global__ __launch_bounds__(kBlock) void k_density(const BlockDesc* __restrict_
                 // Table entries always lie in the scene's range; the clamp keeps
                 // a corrupted table from reading outside the particle arrays.
                 const uint32_t s = max(tab[bk], S.pBegin), e = min(tab[bk + 1], pEnd);
-                for (uint32_t j = s; j < e; ++j) {
+                for (uint32_t js = s; js < e; ++js) {
+                    const uint32_t j = kMap ? pmap[js] : js;
                     if (j == i) continue;
                     const float4 xj = pos[j];
                     const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
@@ -216,19 +227,25 @@ __global__ __launch_bounds__(kBlock) void k_density(const BlockDesc* __restrict_
 // k_force: SF:503-656 for one particle per lane, then the new bucket key and
 // the wave-aggregated histogram (SF:321-331 fused into the integrator's
 // epilogue, as the reference's SF:644-645 already computes the key there).
-template <bool kStatic, bool kDebug, bool kTri>
+// kMap: as k_density's.
+template <bool kStatic, bool kDebug, bool kTri, bool kMap>
 __global__ __launch_bounds__(kBlock) void k_force(const BlockDesc* __restrict__ blocks,
                                                   const SceneDev* __restrict__ scenes,
                                                   const float4* __restrict__ pos, const float4* __restrict__ vel,
                                                   const float* __restrict__ pres, const uint32_t* __restrict__ tab,
                                                   const float4* __restrict__ spos, const uint32_t* __restrict__ stab,
-                                                  const uint32_t* __restrict__ ids, const float* __restrict__ tris,
-                                                  float dt, float4* __restrict__ posT, float4* __restrict__ velT,
+                                                  const uint32_t* __restrict__ pmap, const uint32_t* __restrict__ ids,
+                                                  const float* __restrict__ tris, float dt,
+                                                  float4* __restrict__ posT, float4* __restrict__ velT,
                                                   uint32_t* __restrict__ counts, float4* __restrict__ accOut) {
     const BlockDesc bd = blocks[blockIdx.x];
     const SceneDev& S = scenes[bd.scene];
-    const uint32_t i = bd.begin + threadIdx.x;
-    if (i >= bd.end) return;
+    const uint32_t slot = bd.begin + threadIdx.x;
+    if (slot >= bd.end) return;
+    // A corrupted map entry is clamped, as in k_density; every lane of the wave's
+    // prefix stays active for wave_run below.
+    uint32_t i = kMap ? pmap[slot] : slot;
+    if (kMap && (i < S.pBegin || i >= S.pBegin + S.pCount)) i = S.pBegin;
     const float4 xi = pos[i];
     const float4 vi = vel[i];
     const float pi = pres[i];
@@ -249,7 +266,8 @@ __global__ __launch_bounds__(kBlock) void k_force(const BlockDesc* __restrict__ 
                 // Table entries always lie in the scene's range; the clamp keeps
                 // a corrupted table from reading outside the particle arrays.
                 const uint32_t s = max(tab[bk], S.pBegin), e = min(tab[bk + 1], pEnd);
-                for (uint32_t j = s; j < e; ++j) {
+                for (uint32_t js = s; js < e; ++js) {
+                    const uint32_t j = kMap ? pmap[js] : js;
                     if (j == i) continue;
                     const float4 xj = pos[j];
                     float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
@@ -783,11 +801,14 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* __restrict__ da
 // atomic and keeps its wave order, so afterwards cursor[b] is the bucket start
 // (the reference's final hashMap). The particle state itself moves, so the next
 // step's neighbour reads are contiguous per bucket.
+// pmapOut (particle-map mode, a reordering step: SF:332-345): the state moves
+// into bucket order and the map becomes the identity.
 __device__ inline void scatter_block(uint32_t blk, const BlockDesc* __restrict__ blocks,
                                      const SceneDev* __restrict__ scenes, const float4* __restrict__ posT,
                                      const float4* __restrict__ velT, const uint32_t* __restrict__ idIn,
                                      uint32_t* __restrict__ cursor, float4* __restrict__ pos,
-                                     float4* __restrict__ vel, uint32_t* __restrict__ idOut) {
+                                     float4* __restrict__ vel, uint32_t* __restrict__ idOut,
+                                     uint32_t* __restrict__ pmapOut = nullptr) {
     const BlockDesc bd = blocks[blk];
     const SceneDev& S = scenes[bd.scene];
     const uint32_t tBegin = S.tBegin;
@@ -809,6 +830,7 @@ __device__ inline void scatter_block(uint32_t blk, const BlockDesc* __restrict__
     pos[dst] = p;
     vel[dst] = v;
     idOut[dst] = id;
+    if (pmapOut) pmapOut[dst] = dst;
 }
 
 __global__ __launch_bounds__(kBlock) void k_scatter(const BlockDesc* __restrict__ blocks,
@@ -816,8 +838,38 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const BlockDesc* __restrict_
                                                     const float4* __restrict__ posT, const float4* __restrict__ velT,
                                                     const uint32_t* __restrict__ idIn, uint32_t* __restrict__ cursor,
                                                     float4* __restrict__ pos, float4* __restrict__ vel,
-                                                    uint32_t* __restrict__ idOut) {
-    scatter_block(blockIdx.x, blocks, scenes, posT, velT, idIn, cursor, pos, vel, idOut);
+                                                    uint32_t* __restrict__ idOut, uint32_t* __restrict__ pmapOut) {
+    scatter_block(blockIdx.x, blocks, scenes, posT, velT, idIn, cursor, pos, vel, idOut, pmapOut);
+}
+
+// Particle-map mode, a step that does not reorder (SF:346-356,
+// FillDynamicParticleMapAndFinishHashMap): the state keeps its memory order (this
+// step's output becomes the next step's input at the same index) and slot
+// --cursor[key] of the particle's bucket takes its memory index, the runs of
+// equal keys in a wave taking their slots with one atomicSub as in scatter_block.
+__global__ __launch_bounds__(kBlock) void k_fill_map(const BlockDesc* __restrict__ blocks,
+                                                     const SceneDev* __restrict__ scenes,
+                                                     const float4* __restrict__ posT, const float4* __restrict__ velT,
+                                                     const uint32_t* __restrict__ idIn, uint32_t* __restrict__ cursor,
+                                                     float4* __restrict__ pos, float4* __restrict__ vel,
+                                                     uint32_t* __restrict__ idOut, uint32_t* __restrict__ pmap) {
+    const BlockDesc bd = blocks[blockIdx.x];
+    const SceneDev& S = scenes[bd.scene];
+    const uint32_t i = bd.begin + threadIdx.x;
+    if (i >= bd.end) return;
+    const float4 p = posT[i];
+    const float4 v = velT[i];
+    pos[i] = p;
+    vel[i] = v;
+    idOut[i] = idIn[i];
+    const uint32_t gk = S.tBegin + __float_as_uint(v.w);
+    const Run run = wave_run(gk);
+    uint32_t old = 0;
+    if (run.is_head) old = atomicSub(&cursor[gk], run.len);
+    old = __shfl(old, (int)run.head);
+    const uint32_t dst = old - run.len + run.rank;
+    if (dst < S.pBegin || dst >= S.pBegin + S.pCount) return;  // as in scatter_block
+    pmap[dst] = i;
 }
 
 // ---------------------------------------------------------------------------
@@ -892,11 +944,14 @@ __global__ __launch_bounds__(64) void k_conserved_final(const uint32_t* __restri
 // SimulationEngine.h:115-177 and 229-282 for every dynamic particle: finite
 // state, stored key equal to the key of its position, and the particle inside
 // its bucket's range.
+// In particle-map mode (pmap non-null) bucket slot i holds particle pmap[i], whose
+// stored key must be that bucket's (SimulationEngine.h:266-274).
 __global__ __launch_bounds__(kBlock) void k_validate_particles(const BlockDesc* __restrict__ blocks,
                                                                const SceneDev* __restrict__ scenes,
                                                                const float4* __restrict__ pos,
                                                                const float4* __restrict__ vel,
                                                                const uint32_t* __restrict__ tab,
+                                                               const uint32_t* __restrict__ pmap,
                                                                uint32_t* __restrict__ bits) {
     const BlockDesc bd = blocks[blockIdx.x];
     const SceneDev& S = scenes[bd.scene];
@@ -911,8 +966,13 @@ __global__ __launch_bounds__(kBlock) void k_validate_particles(const BlockDesc* 
     const uint32_t want = fastmod(morton3(cell_coord(p.x, S.h), cell_coord(p.y, S.h), cell_coord(p.z, S.h)),
                                   S.magicM, S.M);
     if (key != want) e |= 0x2u;
-    if (key < S.M) {
-        if (i < tab[S.tBegin + key] || i >= tab[S.tBegin + key + 1]) e |= 0x20u;
+    uint32_t sk = key;  // key of the particle listed in slot i
+    if (pmap) {
+        const uint32_t j = pmap[i];
+        sk = (j >= S.pBegin && j < S.pBegin + S.pCount) ? __float_as_uint(vel[j].w) : 0xFFFFFFFFu;
+    }
+    if (sk < S.M) {
+        if (i < tab[S.tBegin + sk] || i >= tab[S.tBegin + sk + 1]) e |= 0x20u;
     } else {
         e |= 0x20u;
     }
@@ -956,12 +1016,14 @@ __global__ __launch_bounds__(kBlock) void k_validate_static(const SceneDev* __re
 }
 
 // Current state back to original order.
+// outIds[k]: the original index of the particle in bucket slot k (slot k's
+// particle is pmap[k] in particle-map mode, else k).
 __global__ __launch_bounds__(kBlock) void k_unsort(const float4* __restrict__ pos, const float4* __restrict__ vel,
                                                    const uint32_t* __restrict__ ids,
                                                    const uint32_t* __restrict__ origOfSlot, uint32_t n,
                                                    float* __restrict__ outPos, float* __restrict__ outVel,
                                                    float* __restrict__ outP, uint32_t* __restrict__ outKey,
-                                                   uint32_t* __restrict__ outIds) {
+                                                   uint32_t* __restrict__ outIds, const uint32_t* __restrict__ pmap) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const uint32_t o = origOfSlot[ids[k]];
@@ -974,7 +1036,33 @@ __global__ __launch_bounds__(kBlock) void k_unsort(const float4* __restrict__ po
     outVel[3ull * o + 2] = v.z;
     outP[o] = p.w;
     if (outKey) outKey[o] = __float_as_uint(v.w);
-    if (outIds) outIds[k] = o;
+    if (outIds) outIds[k] = pmap ? origOfSlot[ids[pmap[k]]] : o;
+}
+
+// The inverse of k_unsort for a write: slot k takes the position and velocity of
+// its original index from inPos / inVel (either may be null: kept), its pressure
+// (pos.w) and stored key (vel.w) unchanged.
+__global__ __launch_bounds__(kBlock) void k_write(float4* __restrict__ pos, float4* __restrict__ vel,
+                                                  const uint32_t* __restrict__ ids,
+                                                  const uint32_t* __restrict__ origOfSlot, uint32_t n,
+                                                  const float* __restrict__ inPos, const float* __restrict__ inVel) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t o = origOfSlot[ids[k]];
+    if (inPos) {
+        float4 p = pos[k];
+        p.x = inPos[3ull * o + 0];
+        p.y = inPos[3ull * o + 1];
+        p.z = inPos[3ull * o + 2];
+        pos[k] = p;
+    }
+    if (inVel) {
+        float4 v = vel[k];
+        v.x = inVel[3ull * o + 0];
+        v.y = inVel[3ull * o + 1];
+        v.z = inVel[3ull * o + 2];
+        vel[k] = v;
+    }
 }
 
 // Per-particle debug arrays of the last step (indexed in that step's order).

@@ -91,10 +91,20 @@ class SimulationEngineHIP:
                              for s, sc in enumerate(scenes)]
         self._dyn_off = dyn_off.astype(np.int64)
         self._st_off = st_off.astype(np.int64)
+        # A scene file's reorderTimeInterval (SimulationEngineCPU.cpp:400), used in particle-map mode.
+        if len(scenes) == 1 and "reorderTimeInterval" in scenes[0].other:
+            try:
+                self.set_reorder_interval(float(scenes[0].other["reorderTimeInterval"]))
+            except ValueError:
+                pass  # not a number: the reference warns and keeps its interval
         # Boundary meshes (SimulationEngineCPU.cpp:418: triangles = scene.GetMesh().CreateTriangleArray()).
         for s, sc in enumerate(scenes):
             if getattr(sc, "triangles", None) is not None and len(sc.triangles):
                 self.set_triangles(s, sc.triangles)
+
+    def set_reorder_interval(self, interval: float):
+        """reorderTimeInterval of SPH_FLAG_PARTICLE_MAP mode (SimulationEngineCPU.cpp:280-316, 446)."""
+        self._check(self._lib.sph_set_reorder_interval(self._h, ctypes.c_float(interval)))
 
     def Update(self, dt: float, steps: int = 1):
         self._check(self._lib.sph_update(self._h, float(dt), int(steps)))
@@ -133,6 +143,15 @@ class SimulationEngineHIP:
         self._check(self._lib.sph_read(self._h, _ptr(pos, ctypes.c_float), _ptr(vel, ctypes.c_float),
                                        _ptr(p, ctypes.c_float)))
         return pos, vel, p
+
+    def write(self, pos=None, vel=None):
+        """LockWrite + unlock of the current buffer: replace positions and/or
+        velocities (original order) before the next Update; keys and tables are
+        rebuilt by that Update, as in the reference."""
+        n = self.n
+        p = None if pos is None else _f32(pos).reshape(n, 3)
+        v = None if vel is None else _f32(vel).reshape(n, 3)
+        self._check(self._lib.sph_write(self._h, _ptr(p, ctypes.c_float), _ptr(v, ctypes.c_float)))
 
     def read_noalloc(self):
         """read() through sph_read_noalloc: staging in the library's fixed device

@@ -1,6 +1,6 @@
-// Drives SPH::SimulationEngineHIP (include/sph/simulation_engine_hip.hpp) the way
-// the reference's application drives its engines (SimulationVisualisationScene.cpp:
-// Initialize once, Update(dt, 1) per frame, read the dynamic buffer), on a small
+// Drives sph_hip::Engine (include/sph/simulation_engine_hip.hpp) the way the
+// reference's application drives its engines (SimulationVisualisationScene.cpp:
+// Initialize once, Update(dt, 1) per frame, read the particles), on a small
 // dam break given on stdin as: n, then n lines "x y z", then n_s, then n_s lines,
 // then 9 parameter floats. Prints the final state in original order.
 #include <cstdio>
@@ -10,7 +10,7 @@
 
 int main() {
     try {
-        SPH::SceneArrays scene;
+        sph_hip::SceneArrays scene;
         unsigned long long n = 0, ns = 0;
         if (scanf("%llu", &n) != 1) return 2;
         scene.dynamicPositions.resize(n);
@@ -27,21 +27,20 @@ int main() {
         float dt = 0;
         if (scanf("%d %f", &frames, &dt) != 2) return 2;
 
-        SPH::SimulationEngineHIP engine(0);
-        SPH::SimulationEngine& e = engine;
+        sph_hip::Engine e(0);
         e.Initialize(scene);
         // Frames as the reference's app runs them, with the asynchronous
         // readback a renderer would use: enqueue a copy each frame, consume it
         // when it has landed without waiting for the GPU.
-        SPH::HIPAsyncBufferManager* async = engine.GetAsyncBufferManager();
+        sph_hip::AsyncReadback* async = e.GetReadback();
         unsigned landed = 0;
         for (int i = 0; i < frames; ++i) {
             e.Update(dt, 1);
             async->Advance();
-            const std::vector<SPH::DynamicParticle>* view = nullptr;
+            const std::vector<sph_hip::Record>* view = nullptr;
             if (async->TryLockRead(view) && view->size() == n) ++landed;
         }
-        SPH::ParticleBufferManager* m = e.GetDynamicParticlesBufferManager();
+        sph_hip::AsyncReadback* m = e.GetReadback();
         m->FlushAllOperations();
         const auto& recs = m->LockRead();
         fprintf(stderr, "frames with a landed copy: %u of %d\n", landed, frames);

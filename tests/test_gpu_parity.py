@@ -20,9 +20,9 @@ import oracle as orc
 from helpers import bucket_multisets_equal, check_buckets, expected_starts, max_norm_rel, max_rel
 from sphfluidsimulation_amd import SPHError, SimulationEngineHIP, scenes
 from sphfluidsimulation_amd._lib import (SPH_E_STATE, SPH_FLAG_CELL_WALK, SPH_FLAG_CW_MASKS, SPH_FLAG_FP16_OFFSETS,
-                                         SPH_FLAG_HIP_GRAPH,
+                                         SPH_FLAG_HIP_GRAPH, SPH_FLAG_PARTICLE_MAP,
                                          SPH_FLAG_RECORD_DEBUG, SPH_FLAG_RUN_WALK, SPH_FLAG_TILE_WALK,
-                                         SPH_V_NONFINITE)
+                                         SPH_V_KEY, SPH_V_NONFINITE)
 
 pytestmark = pytest.mark.gpu
 
@@ -433,6 +433,99 @@ def test_nan_positions(kind, walk):
     ref.close()
     # The poison spreads: more NaN particles after the second step than the scene's.
     assert counts[1] >= counts[0] > 0
+
+
+@pytest.mark.parametrize("walk", WALKS)
+def test_write_then_step(s16k, walk):
+    """sph_write, the LockWrite of the current buffer: 200 particles moved into other
+    cells and new velocities, written into the GPU engine and the oracle alike. Until
+    the next step rehashes, keys and tables stay those of the last sort, so the
+    validators report SPH_V_KEY (SimulationEngine.h:115-282); the step walks those
+    tables, as the reference's does, and meets the one-step bars; then every
+    invariant holds again."""
+    say(f"write then step (walk flags {walk})")
+    sc = s16k
+    rng = np.random.default_rng(17)
+    pos = sc.pos.copy()
+    pos[rng.choice(sc.n, 200, replace=False)] += np.float32([0.12, -0.03, 0.07])
+    vel = rng.normal(0, 0.3, size=pos.shape).astype(np.float32)
+    e = SimulationEngineHIP(0)
+    e.Initialize(sc, flags=SPH_FLAG_RECORD_DEBUG | walk)
+    ref = orc.Oracle(sc.params, sc.pos, sc.vel, sc.static_pos, threads=4, flags=orc.FLAG_RECORD_DEBUG)
+    keys0 = e.read_keys()
+    e.write(pos, vel)
+    ref.write(pos, vel)
+    p1, v1, _ = e.read()
+    np.testing.assert_array_equal(p1, pos)
+    np.testing.assert_array_equal(v1, vel)
+    np.testing.assert_array_equal(e.read_keys(), keys0)
+    assert e.validate() == SPH_V_KEY
+    e.Update(1e-3, 1)
+    ref.update(1e-3, 1)
+    _check_step(sc, (*e.read_debug(), *e.read()), (*ref.read_debug(), *ref.read()), 1e-3)
+    _step_integer_parity(e, sc)
+    e.close()
+    ref.close()
+
+
+def _map_mode_run(sc, interval, updates, dt=1e-3, threads=4):
+    """SPH_FLAG_PARTICLE_MAP against the oracle with the same reorder interval: the
+    first update from the same initial state side by side; every later update
+    against an oracle seeded with the engine's state before it (the two memory
+    orders differ, the physics must not). After each update the engine's keys,
+    table and particle-map buckets are bit-exact for its own positions."""
+    e = SimulationEngineHIP(0)
+    e.Initialize(sc, flags=SPH_FLAG_RECORD_DEBUG | SPH_FLAG_PARTICLE_MAP)
+    e.set_reorder_interval(interval)
+    ref = orc.Oracle(sc.params, sc.pos, sc.vel, sc.static_pos, threads=threads, flags=orc.FLAG_RECORD_DEBUG)
+    ref.set_reorder_interval(interval)
+    e.Update(dt, 1)
+    ref.update(dt, 1)
+    _check_step(sc, (*e.read_debug(), *e.read()), (*ref.read_debug(), *ref.read()), dt)
+    ref.close()
+    _step_integer_parity(e, sc)
+    for _ in range(updates - 1):
+        pos, vel, _ = e.read()
+        e.Update(dt, 1)
+        ref = orc.Oracle(sc.params, pos, vel, sc.static_pos, threads=threads, flags=orc.FLAG_RECORD_DEBUG)
+        ref.update(dt, 1)
+        _check_step(sc, (*e.read_debug(), *e.read()), (*ref.read_debug(), *ref.read()), dt)
+        ref.close()
+        _step_integer_parity(e, sc)
+    e.close()
+
+
+@pytest.mark.parametrize("interval", [pytest.param(3.4e38, id="never-reorder"),
+                                      pytest.param(-1.0, id="reorder-every-update")])
+def test_particle_map_mode_16k(s16k, interval):
+    say(f"particle-map mode 16k, reorder interval {interval}")
+    _map_mode_run(s16k, interval, 6)
+
+
+def test_particle_map_mode_1m(s1m):
+    say("particle-map mode 1M, reorder interval FLT_MAX")
+    _map_mode_run(s1m, 3.4e38, 3, threads=16)
+
+
+def test_particle_map_mode_batched():
+    """Two scenes in one engine in particle-map mode: each against its oracle."""
+    say("particle-map mode, batch of two scenes")
+    scs = scenes.batch_scenes(2, 3000, 6000, seed=31)
+    e = SimulationEngineHIP(0)
+    e.Initialize(scs, flags=SPH_FLAG_RECORD_DEBUG | SPH_FLAG_PARTICLE_MAP)
+    e.Update(1e-3, 1)
+    d, a = e.read_debug()
+    pos, vel, p = e.read()
+    off = 0
+    for sc in scs:
+        ref = orc.Oracle(sc.params, sc.pos, sc.vel, sc.static_pos, threads=4, flags=orc.FLAG_RECORD_DEBUG)
+        ref.update(1e-3, 1)
+        sl = slice(off, off + sc.n)
+        _check_step(sc, (d[sl], a[sl], pos[sl], vel[sl], p[sl]), (*ref.read_debug(), *ref.read()), 1e-3)
+        ref.close()
+        off += sc.n
+    assert e.validate() == 0
+    e.close()
 
 
 def test_autotune_leaves_state_unchanged(s16k):

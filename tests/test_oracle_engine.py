@@ -52,13 +52,17 @@ def visit_counts(cells, M, keys_other):
     return out
 
 
-def brute_density(params, pos, spos, M, Ms):
-    """Density of every particle from all pairs, weighted by bucket-visit counts."""
+def brute_density(params, pos, spos, M, Ms, keys=None):
+    """Density of every particle from all pairs, weighted by bucket-visit counts.
+    `keys`: the bucket each particle is listed in, when the table is not the one of
+    the current positions (after a write); by default the keys of `pos`."""
     h = float(np.float32(params.maxInteractionDistance))  # the engines take float32 parameters
     m = float(np.float32(params.particleMass))
     C = 15.0 / (3.1415 * h ** 6)
     self_d = m * h ** 3 * C
     k_dyn, cells = np_keys(pos, h, M)
+    if keys is not None:
+        k_dyn = np.asarray(keys, dtype=np.int64)
     P = pos.astype(np.float64)
     r = np.sqrt(((P[:, None, :] - P[None, :, :]) ** 2).sum(-1))
     w = np.where(r < h, (h - r) ** 3, 0.0)
@@ -353,15 +357,17 @@ def test_wall_acceleration_closed_form(rest_density):
         assert max_norm_rel(a[0], want) < 1e-12
 
 
-def brute_accel(prm, pos, vel, spos, M, Ms):
+def brute_accel(prm, pos, vel, spos, M, Ms, keys=None):
     """Acceleration of every particle from all pairs, each pair weighted by how many of
     the particle's 27 bucket visits reach the other particle's bucket (the density of
-    brute_density supplies p_i and rho_i)."""
+    brute_density supplies p_i and rho_i); `keys` as for brute_density."""
     h, m, C, k, rho0, mu, g = _consts(prm)
-    rho = brute_density(prm, pos, spos, M, Ms)
+    rho = brute_density(prm, pos, spos, M, Ms, keys)
     pr = k * (rho - rho0)
     rho_i = pr / k + rho0
     k_dyn, cells = np_keys(pos, h, M)
+    if keys is not None:
+        k_dyn = np.asarray(keys, dtype=np.int64)
     P, V = pos.astype(np.float64), vel.astype(np.float64)
     n = P.shape[0]
     d = P[None, :, :] - P[:, None, :]  # d[i, j] = x_j - x_i
@@ -408,6 +414,38 @@ def test_accel_brute_force(hash_size):
     f = orc.Oracle(sc.params, sc.pos, v, sc.static_pos, threads=2, hash_size=hash_size, flags=orc.FLAG_RECORD_DEBUG)
     f.update(1e-3, 1)
     assert max_norm_rel(f.read_debug()[1], want) < 1e-5
+
+
+def test_write_then_step_walks_the_last_tables():
+    """A write through LockWrite (OfflineCPUParticleBufferManager.cpp:68-74) replaces
+    positions and velocities but not the stored hashes or the tables, so the next
+    step finds each particle in the bucket it was listed in before the write, while
+    each particle walks the 27 cells around its written position: the brute force
+    with the pre-write keys. That step then rehashes from the new positions."""
+    sc = small_scene(wall_gap=0.04)
+    rng = np.random.default_rng(13)
+    o = orc.Oracle(sc.params, sc.pos, sc.vel, sc.static_pos, threads=2, fp64=True, flags=orc.FLAG_RECORD_DEBUG)
+    keys0 = o.read_keys()
+    pos = sc.pos.copy()
+    moved = rng.choice(sc.n, 40, replace=False)
+    pos[moved] += np.float32([0.12, -0.03, 0.07])  # into other cells
+    vel = rng.normal(0, 0.3, size=pos.shape).astype(np.float32)
+    o.write(pos, vel)
+    p1, v1, _ = o.read()
+    np.testing.assert_array_equal(p1, pos)
+    np.testing.assert_array_equal(v1, vel)
+    np.testing.assert_array_equal(o.read_keys(), keys0)
+    assert np.any(np_keys(pos, 0.1, o.M)[0] != keys0)
+    o.update(1e-3, 1)
+    d, a = o.read_debug()
+    assert max_rel(d, brute_density(sc.params, pos, sc.static_pos, o.M, sc.n_static, keys0)) < 1e-12
+    assert max_norm_rel(a, brute_accel(sc.params, pos, vel, sc.static_pos, o.M, sc.n_static, keys0)) < 1e-10
+    # with the fresh keys the sums differ: the stale table is what was walked
+    assert max_rel(d, brute_density(sc.params, pos, sc.static_pos, o.M, sc.n_static)) > 1e-3
+    p2, _, _ = o.read()
+    keys = o.read_keys()
+    np.testing.assert_array_equal(keys, np_keys(p2.astype(np.float32), 0.1, o.M)[0])
+    check_buckets(o.read_hash_map(), o.read_bucket_ids(), keys)
 
 
 def test_fp32_tracks_fp64():
