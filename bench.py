@@ -38,6 +38,9 @@ CONFIGS = {
     "batch256-plain": ("batch_256", 1e-3, 500, {}),
     # 64 scenes in mixed precision.
     "batch64": ("batch_64_mixed", 1e-3, 200, {"graph": True, "mixed": True}),
+    # The scene the reference's application loads (triangleTestScene.json with its
+    # 1,096-triangle mesh), stepped as the application steps it.
+    "triangle": ("triangle_test_scene", 1e-2, 200, {}),
 }
 
 
@@ -198,7 +201,12 @@ def main():
     args.mixed = args.mixed or stated.get("mixed", False)
     steps = args.steps if args.steps is not None else default_steps
     t0 = time.perf_counter()
-    built = getattr(scenes, factory)()
+    if factory == "triangle_test_scene":
+        from sphfluidsimulation_amd import scene_json
+        built = scene_json.load_scene(os.path.join(ROOT, "tests", "golden", "app", "assets", "simulationScenes",
+                                                   "triangleTestScene.json"))
+    else:
+        built = getattr(scenes, factory)()
     scene_list = built if isinstance(built, list) else [built]
     log(f"[bench] {args.config}: {len(scene_list)} scene(s), "
         f"{sum(s.n for s in scene_list)} particles, generated in {time.perf_counter() - t0:.1f} s")

@@ -943,15 +943,7 @@ __device__ inline void force_finish(ForceSums<kMixed, kVisc>& f, const SceneDev&
     float Fx = f.fpx.s * sP + f.fvx.s * sV + fsx.s * sP + (-vi.x * sumD2s) * sVs;
     float Fy = f.fpy.s * sP + f.fvy.s * sV + fsy.s * sP + (-vi.y * sumD2s) * sVs;
     float Fz = f.fpz.s * sP + f.fvz.s * sV + fsz.s * sP + (-vi.z * sumD2s) * sVs;
-    if (kTri) {
-        const float3 p3 = make_float3(xi.x, xi.y, xi.z);
-        for (uint32_t t = 0; t < S.triCount; ++t) {
-            const float3 fo = triangle_force(p3, tris + 9u * (S.triBegin + t));
-            Fx += fo.x;
-            Fy += fo.y;
-            Fz += fo.z;
-        }
-    }
+    if (kTri) mesh_force(make_float3(xi.x, xi.y, xi.z), S, tris, Fx, Fy, Fz);
     const float nan = __builtin_nanf("");
     const float ax = poisoned ? nan : Fx / rho + S.gx, ay = poisoned ? nan : Fy / rho + S.gy,
                 az = poisoned ? nan : Fz / rho + S.gz;

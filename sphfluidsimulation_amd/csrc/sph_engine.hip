@@ -253,6 +253,92 @@ void counting_sort(const std::vector<uint32_t>& keys, uint64_t M, std::vector<ui
     for (uint64_t i = 0; i < n; ++i) slotOf[i] = --table[keys[i]];
 }
 
+// Bins of one scene's triangles for mesh_force. The grid covers the triangles'
+// bounding boxes widened by the force's range (SF:205, 2.5) plus a margin for the
+// rounding of the device's distance and bin index; bins have an edge of half the
+// range (larger when that would make more than 2^21 bins), and each lists, in
+// index order, the triangles whose widened box meets it. Appends the bin-start
+// table and the lists to `tables` and records the grid in S (offsets relative to
+// `tableBase`, the float offset the tables will have in the mesh buffer).
+void build_mesh_bins(const float* tri9, uint64_t count, SceneDev& S, std::vector<uint32_t>& tables,
+                     uint64_t tableBase) {
+    S.nbx = S.nby = S.nbz = 0;
+    if (count == 0) return;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, maxAbs = 0.0;
+    for (uint64_t k = 0; k < 9 * count; ++k) {
+        const double v = tri9[k];
+        lo[k % 3] = std::min(lo[k % 3], v);
+        hi[k % 3] = std::max(hi[k % 3], v);
+        maxAbs = std::max(maxAbs, std::fabs(v));
+    }
+    const double widen = 2.5 * 1.001 + 1e-4 * (1.0 + maxAbs);
+    double edge = 1.25;
+    uint64_t n[3];
+    for (;;) {
+        uint64_t total = 1;
+        for (int a = 0; a < 3; ++a) {
+            n[a] = std::max<uint64_t>(1, (uint64_t)std::ceil((hi[a] - lo[a] + 2.0 * widen) / edge));
+            total *= n[a];
+        }
+        if (total <= (1ull << 21)) break;
+        edge *= 1.25;
+    }
+    const double o[3] = {lo[0] - widen, lo[1] - widen, lo[2] - widen};
+    S.bx0 = (float)o[0];
+    S.by0 = (float)o[1];
+    S.bz0 = (float)o[2];
+    S.binInv = (float)(1.0 / edge);
+    S.nbx = (uint32_t)n[0];
+    S.nby = (uint32_t)n[1];
+    S.nbz = (uint32_t)n[2];
+    const uint64_t nb = n[0] * n[1] * n[2];
+    // The bins each triangle's widened box meets, in the float32 grid the device
+    // uses (origin and edge as stored), so the two agree on every boundary.
+    const double fo[3] = {S.bx0, S.by0, S.bz0}, finv = S.binInv;
+    auto range = [&](uint64_t t, int a, uint64_t& b0, uint64_t& b1) {
+        double mn = 1e300, mx = -1e300;
+        for (int v = 0; v < 3; ++v) {
+            mn = std::min(mn, (double)tri9[9 * t + 3 * v + a]);
+            mx = std::max(mx, (double)tri9[9 * t + 3 * v + a]);
+        }
+        const double g0 = std::floor((mn - widen - fo[a]) * finv), g1 = std::floor((mx + widen - fo[a]) * finv);
+        b0 = (uint64_t)std::max(0.0, g0);
+        b1 = (uint64_t)std::min((double)(n[a] - 1), std::max(0.0, g1));
+    };
+    std::vector<uint32_t> cnt(nb + 1, 0u);
+    for (int pass = 0; pass < 2; ++pass) {
+        std::vector<uint32_t> fill;
+        if (pass == 1) {
+            uint32_t sum = 0;
+            for (uint64_t b = 0; b <= nb; ++b) {
+                const uint32_t c = cnt[b];
+                cnt[b] = sum;
+                sum += c;
+            }
+            fill.assign(cnt.begin(), cnt.end() - 1);
+            S.binStart = (uint32_t)(tableBase + tables.size());
+            tables.insert(tables.end(), cnt.begin(), cnt.end());
+            S.binList = (uint32_t)(tableBase + tables.size());
+            tables.resize(tables.size() + cnt[nb]);
+        }
+        for (uint64_t t = 0; t < count; ++t) {
+            uint64_t x0, x1, y0, y1, z0, z1;
+            range(t, 0, x0, x1);
+            range(t, 1, y0, y1);
+            range(t, 2, z0, z1);
+            for (uint64_t z = z0; z <= z1; ++z)
+                for (uint64_t y = y0; y <= y1; ++y)
+                    for (uint64_t x = x0; x <= x1; ++x) {
+                        const uint64_t b = (z * n[1] + y) * n[0] + x;
+                        if (pass == 0)
+                            ++cnt[b];
+                        else
+                            tables[S.binList - tableBase + fill[b]++] = (uint32_t)t;
+                    }
+        }
+    }
+}
+
 uint32_t key_of(const float* p, float h, uint64_t M) {
     const uint32_t code = morton3(cell_coord(p[0], h), cell_coord(p[1], h), cell_coord(p[2], h));
     return (uint32_t)(code % M);
@@ -877,11 +963,22 @@ int sph_set_triangles(sph_handle h, uint32_t scene, const float* tri9, uint64_t 
     h->hTris.swap(all);
     h->nTris = h->hTris.size() / 9;
     h->hasTris = h->nTris > 0;
+    // The mesh buffer: every scene's triangles, then every scene's bin tables.
+    std::vector<uint32_t> tables;
+    for (uint32_t s = 0; s < h->nScenes; ++s)
+        build_mesh_bins(h->hTris.data() + 9ull * S[s].triBegin, S[s].triCount, S[s], tables, h->hTris.size());
+    if (h->hTris.size() + tables.size() >= (1ull << 32)) {
+        h->err = "sph_set_triangles: mesh tables above 2^32 entries";
+        return SPH_E_INVALID;
+    }
+    std::vector<float> mesh(h->hTris);
+    mesh.resize(h->hTris.size() + tables.size());
+    if (!tables.empty()) memcpy(mesh.data() + h->hTris.size(), tables.data(), sizeof(uint32_t) * tables.size());
     dfree(h->tris);
     int rc;
-    if ((rc = dalloc(h, &h->tris, std::max<uint64_t>(h->hTris.size(), 1)))) return rc;
-    if (h->nTris)
-        SPH_HIP(h, hipMemcpy(h->tris, h->hTris.data(), sizeof(float) * h->hTris.size(), hipMemcpyHostToDevice));
+    if ((rc = dalloc(h, &h->tris, std::max<uint64_t>(mesh.size(), 1)))) return rc;
+    if (!mesh.empty())
+        SPH_HIP(h, hipMemcpy(h->tris, mesh.data(), sizeof(float) * mesh.size(), hipMemcpyHostToDevice));
     SPH_HIP(h, hipMemcpy(h->dScenes, S.data(), sizeof(SceneDev) * h->nScenes, hipMemcpyHostToDevice));
     for (auto& g : h->graph) {
         if (g) (void)hipGraphExecDestroy(g);

@@ -37,6 +37,12 @@ struct alignas(16) SceneDev {
     uint64_t magicMs;
     uint32_t triBegin, triCount;
     uint32_t wmBegin, wmMask;  // wall-cell bitmap: words [wmBegin, wmBegin + wmMask]
+    // Boundary-mesh bins (mesh_force): a grid of nbx x nby x nbz bins of edge
+    // 1 / binInv from (bx0, by0, bz0); the bin-start table at float offset binStart
+    // of the mesh buffer, the triangle lists at float offset binList (uint bits).
+    float bx0, by0, bz0, binInv;
+    uint32_t nbx, nby, nbz, binStart;
+    uint32_t binList, pad0, pad1, pad2;
 };
 
 struct alignas(16) BlockDesc {
@@ -153,6 +159,36 @@ __device__ inline float3 triangle_force(float3 p, const float* t) {
     const float tt = 1.0f / (dist / 2.5f) - 1.0f;
     const float amp = tt * tt * 50.0f;
     return make_float3(-d.x / dist * amp, -d.y / dist * amp, -d.z / dist * amp);
+}
+
+// The boundary-mesh force of SF:636-637, the scene's triangles summed in index
+// order onto (fx, fy, fz), over the triangles of the particle's bin only. A bin
+// lists, in index order, every triangle whose bounding box widened by the force's
+// range (2.5, plus a rounding margin) meets the bin, so each triangle left out is
+// farther than the range from the particle and adds an exact zero in the
+// reference (SF:211-213); a point outside the grid has no triangle in range. A
+// particle with a NaN coordinate takes every triangle, as the reference does.
+// `mesh` holds the triangles (9 floats each) followed by the bin tables.
+__device__ inline void mesh_force(float3 p, const SceneDev& S, const float* __restrict__ mesh, float& fx, float& fy,
+                                  float& fz) {
+    const float* tri = mesh + 9u * S.triBegin;
+    auto add = [&](uint32_t t) {
+        const float3 f = triangle_force(p, tri + 9u * t);
+        fx += f.x;
+        fy += f.y;
+        fz += f.z;
+    };
+    if (p.x != p.x || p.y != p.y || p.z != p.z) {
+        for (uint32_t t = 0; t < S.triCount; ++t) add(t);
+        return;
+    }
+    const float gx = floorf((p.x - S.bx0) * S.binInv), gy = floorf((p.y - S.by0) * S.binInv),
+                gz = floorf((p.z - S.bz0) * S.binInv);
+    if (!(gx >= 0.f && gy >= 0.f && gz >= 0.f && gx < (float)S.nbx && gy < (float)S.nby && gz < (float)S.nbz)) return;
+    const uint32_t b = ((uint32_t)gz * S.nby + (uint32_t)gy) * S.nbx + (uint32_t)gx;
+    const uint32_t* start = reinterpret_cast<const uint32_t*>(mesh + S.binStart);
+    const uint32_t* list = reinterpret_cast<const uint32_t*>(mesh + S.binList);
+    for (uint32_t k = start[b], e = start[b + 1]; k < e; ++k) add(list[k]);
 }
 
 constexpr uint32_t kMaskBits = 256;  // candidates per particle recorded by the density pass
@@ -336,15 +372,7 @@ __global__ __launch_bounds__(kBlock) void k_force(const BlockDesc* __restrict__ 
     float Fx = fpx * sP + fvx * sV + fsx * sP + (-vi.x * sumD2s) * sVs;
     float Fy = fpy * sP + fvy * sV + fsy * sP + (-vi.y * sumD2s) * sVs;
     float Fz = fpz * sP + fvz * sV + fsz * sP + (-vi.z * sumD2s) * sVs;
-    if (kTri) {
-        const float3 p3 = make_float3(xi.x, xi.y, xi.z);
-        for (uint32_t t = 0; t < S.triCount; ++t) {
-            const float3 f = triangle_force(p3, tris + 9u * (S.triBegin + t));
-            Fx += f.x;
-            Fy += f.y;
-            Fz += f.z;
-        }
-    }
+    if (kTri) mesh_force(make_float3(xi.x, xi.y, xi.z), S, tris, Fx, Fy, Fz);
     const float ax = Fx / rho + S.gx, ay = Fy / rho + S.gy, az = Fz / rho + S.gz;
     if (kDebug) accOut[i] = make_float4(ax, ay, az, 0.f);
     // SF:641-645: symplectic Euler, then the key of the new cell.
@@ -600,15 +628,7 @@ __global__ __launch_bounds__(kBlock) void k_force_runs(const BlockDesc* __restri
     float Fx = fpx * sP + fvx * sV + fsx * sP + (-vi.x * sumD2s) * sVs;
     float Fy = fpy * sP + fvy * sV + fsy * sP + (-vi.y * sumD2s) * sVs;
     float Fz = fpz * sP + fvz * sV + fsz * sP + (-vi.z * sumD2s) * sVs;
-    if (kTri) {
-        const float3 p3 = make_float3(xi.x, xi.y, xi.z);
-        for (uint32_t t = 0; t < S.triCount; ++t) {
-            const float3 f = triangle_force(p3, tris + 9u * (S.triBegin + t));
-            Fx += f.x;
-            Fy += f.y;
-            Fz += f.z;
-        }
-    }
+    if (kTri) mesh_force(make_float3(xi.x, xi.y, xi.z), S, tris, Fx, Fy, Fz);
     const float ax = Fx / rho + S.gx, ay = Fy / rho + S.gy, az = Fz / rho + S.gz;
     if (kDebug) accOut[i] = make_float4(ax, ay, az, 0.f);
     const float nvx = vi.x + ax * dt, nvy = vi.y + ay * dt, nvz = vi.z + az * dt;

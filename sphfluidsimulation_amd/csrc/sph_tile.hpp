@@ -416,15 +416,7 @@ __global__ __launch_bounds__(kBlock) void k_force_tile(const BlockDesc* __restri
     float Fx = fpx * sP + fvx * sV + fsx * sP + (-vi.x * sumD2s) * sVs;
     float Fy = fpy * sP + fvy * sV + fsy * sP + (-vi.y * sumD2s) * sVs;
     float Fz = fpz * sP + fvz * sV + fsz * sP + (-vi.z * sumD2s) * sVs;
-    if (kTri) {
-        const float3 p3 = make_float3(xi.x, xi.y, xi.z);
-        for (uint32_t t = 0; t < S.triCount; ++t) {
-            const float3 f = triangle_force(p3, tris + 9u * (S.triBegin + t));
-            Fx += f.x;
-            Fy += f.y;
-            Fz += f.z;
-        }
-    }
+    if (kTri) mesh_force(make_float3(xi.x, xi.y, xi.z), S, tris, Fx, Fy, Fz);
     const float ax = Fx / rho + S.gx, ay = Fy / rho + S.gy, az = Fz / rho + S.gz;
     if (kDebug) accOut[i] = make_float4(ax, ay, az, 0.f);
     // SF:641-645: symplectic Euler, then the key of the new cell.

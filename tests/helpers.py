@@ -46,3 +46,24 @@ def max_norm_rel(a, b) -> float:
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def point_triangle_distances(p, tris):
+    """FP64 distance from each point to each triangle: to the plane where the
+    projection falls inside the triangle, else to the nearest edge (a formulation
+    independent of the closest-point region tests of SPHFunctions.cpp:156-201)."""
+    t = tris.astype(np.float64).reshape(-1, 3, 3)
+    a, b, c = t[:, 0], t[:, 1], t[:, 2]
+    n = np.cross(b - a, c - a)
+    P = p.astype(np.float64)[:, None, :]
+    d_plane = np.abs(((P - a) * n).sum(-1)) / np.linalg.norm(n, axis=-1)
+    q = P - (((P - a) * n).sum(-1) / (n * n).sum(-1))[..., None] * n
+    inside = np.ones(d_plane.shape, bool)
+    for u, v in ((a, b), (b, c), (c, a)):
+        inside &= (np.cross(v - u, q - u) * n).sum(-1) >= 0
+    def seg(u, v):
+        w = v - u
+        s = np.clip(((P - u) * w).sum(-1) / (w * w).sum(-1), 0.0, 1.0)
+        return np.linalg.norm(P - (u + s[..., None] * w), axis=-1)
+    d_edge = np.minimum(np.minimum(seg(a, b), seg(b, c)), seg(c, a))
+    return np.where(inside, np.minimum(d_plane, d_edge), d_edge)

@@ -13,11 +13,14 @@ point results are compared to these tolerances, not bit for bit.
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 import pytest
 
 import oracle as orc
-from helpers import bucket_multisets_equal, check_buckets, expected_starts, max_norm_rel, max_rel
+from helpers import (bucket_multisets_equal, check_buckets, expected_starts, max_norm_rel, max_rel,
+                     point_triangle_distances)
 from sphfluidsimulation_amd import SPHError, SimulationEngineHIP, scenes
 from sphfluidsimulation_amd._lib import (SPH_E_STATE, SPH_FLAG_CELL_WALK, SPH_FLAG_CW_MASKS, SPH_FLAG_FP16_OFFSETS,
                                          SPH_FLAG_HIP_GRAPH, SPH_FLAG_PARTICLE_MAP,
@@ -628,6 +631,91 @@ def test_scene_file_with_mesh_one_step(tmp_path):
     e.close()
     ref.close()
     _check_step(sc, got, want, 1e-2)
+
+
+GOLDEN_SCENES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "app", "assets",
+                             "simulationScenes")
+
+
+def test_reference_triangle_test_scene():
+    """The scene the reference's application loads (triangleTestScene.json: a
+    FilledBox of 9,261 particles at 20 per unit without jitter, h = 2, and the
+    1,096-triangle boundary mesh untitled.obj; SimulationVisualisationScene.cpp:145-171
+    steps it at dt = 0.01, one step per frame), from the reference's own files (copied
+    by tests/golden/copy_scene_assets.py): one frame against the oracle, which sums
+    all 1,096 triangles for every particle (SF:636-637), then four more frames and the
+    integer state."""
+    from sphfluidsimulation_amd import scene_json
+    sc = scene_json.load_scene(os.path.join(GOLDEN_SCENES, "triangleTestScene.json"))
+    assert sc.n == 9261 and sc.triangles.shape == (1096, 9) and sc.params.maxInteractionDistance == 2.0
+    say("triangleTestScene.json: one frame against the oracle")
+    e = SimulationEngineHIP(0)
+    e.Initialize(sc, flags=SPH_FLAG_RECORD_DEBUG)
+    ref = orc.Oracle(sc.params, sc.pos, sc.vel, sc.static_pos, threads=4, flags=orc.FLAG_RECORD_DEBUG)
+    ref.set_triangles(sc.triangles)
+    e.Update(1e-2, 1)
+    ref.update(1e-2, 1)
+    _check_step(sc, (*e.read_debug(), *e.read()), (*ref.read_debug(), *ref.read()), 1e-2)
+    ref.close()
+    e.Update(1e-2, 4)
+    _step_integer_parity(e, sc)
+    e.close()
+
+
+@pytest.mark.parametrize("walk", WALKS)
+def test_mesh_bins_keep_every_triangle_in_range(walk):
+    """Particles 2.05 apart (no neighbour within h = 2) scattered over the box of
+    the reference's mesh widened by the force's range, without gravity, so each
+    acceleration is that particle's mesh force over its self density: against the
+    oracle's sum over every triangle (SF:636-637), row by row. A triangle at
+    distance d < 2.5 adds 50 tt^2 along the direction to it, tt = 2.5/d - 1
+    (SF:202-223). The two float32 computations of d (closest point minus the
+    particle, coordinates up to about 30) differ by up to about ten ulps, dd =
+    2e-5 absolute, which moves tt by 2.5 dd / d^2 and the direction by dd / d; so
+    each row is held to 1e-5 of itself plus, over its triangles in range,
+    100 tt dtt + 50 dtt^2 + 100 tt^2 dd / d. A triangle left out of a bin would
+    change its row by its whole term, 50 tt^2, which exceeds its share of that
+    budget for any d below 2.4999 (tt > 4e-5)."""
+    say(f"mesh bins against the full triangle loop (walk flags {walk})")
+    from sphfluidsimulation_amd import scene_json
+    tris = scene_json.load_scene(os.path.join(GOLDEN_SCENES, "triangleTestScene.json")).triangles
+    v = tris.reshape(-1, 3, 3)
+    lo, hi = v.min((0, 1)) - 3.0, v.max((0, 1)) + 3.0
+    rng = np.random.default_rng(41)
+    dims = np.ceil((hi - lo) / 2.05).astype(int)
+    grid = np.stack(np.meshgrid(*[np.arange(d) for d in dims], indexing="ij"), -1).reshape(-1, 3)
+    pos = (lo + (grid + 0.5) * 2.05 + rng.uniform(-0.01, 0.01, size=grid.shape)).astype(np.float32)
+    prm = scenes.Params(1.0, 200.0, 1.0, 0.1, 0.0, 0.0, 0.0, 1.0, 2.0)
+    sc = scenes.Scene(prm, pos, np.zeros_like(pos), np.zeros((0, 3), np.float32), name="mesh-bins", triangles=tris)
+    e = SimulationEngineHIP(0)
+    e.Initialize(sc, flags=SPH_FLAG_RECORD_DEBUG | walk)
+    ref = orc.Oracle(sc.params, sc.pos, sc.vel, None, threads=4, flags=orc.FLAG_RECORD_DEBUG)
+    ref.set_triangles(tris)
+    e.Update(1e-3, 1)
+    ref.update(1e-3, 1)
+    _, a = e.read_debug()
+    _, ra = ref.read_debug()
+    e.close()
+    ref.close()
+    row = np.abs(ra).max(axis=1)
+    near = row > 0
+    say(f"{int(near.sum())} of {sc.n} particles within range of the mesh")
+    assert near.sum() > 100 and (~near).sum() > 100
+    assert np.all(a[~near] == 0.0)
+    rho = float(np.float32(prm.particleMass)) * 2.0 ** 3 * 15.0 / (3.1415 * 2.0 ** 6)  # self density
+    budget = np.zeros(sc.n)
+    for k in range(0, sc.n, 256):
+        d = point_triangle_distances(pos[k:k + 256], tris)
+        inr = d <= 2.5 + 1e-4
+        dd = 2e-5
+        tt = np.where(inr, np.maximum(2.5 / d - 1.0, 0.0), 0.0)
+        dtt = np.where(inr, 2.5 * dd / d ** 2, 0.0)
+        pair = 100.0 * tt * dtt + 50.0 * dtt ** 2 + np.where(inr, 100.0 * tt ** 2 * dd / d, 0.0)
+        budget[k:k + 256] = pair.sum(axis=1) / rho
+    err = np.abs(a - ra).max(axis=1)
+    bar = 1e-5 * row + budget
+    say(f"worst row: |da| {float(err.max()):.3e}, |da| / bar {float((err / np.maximum(bar, 1e-30)).max()):.3f}")
+    assert np.all(err <= bar)
 
 
 def test_async_readback_ring(s16k):

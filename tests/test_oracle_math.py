@@ -137,3 +137,31 @@ def test_triangle_force_geometry():
     dist = math.sqrt(2.0)
     amp = 50.0 * (2.5 / dist - 1.0) ** 2
     np.testing.assert_allclose(f, [amp / dist, amp / dist, 0.0], rtol=1e-5)
+
+
+def test_triangle_force_magnitude_matches_point_triangle_distance():
+    """SPHFunctions.cpp:202-223: |F| = 50 (2.5/d - 1)^2 for d < 2.5 and 0 beyond,
+    with d the point-triangle distance; d here is computed independently (plane or
+    nearest edge, helpers.point_triangle_distances) for random points around the
+    reference's own mesh."""
+    import os
+    from helpers import point_triangle_distances
+    from sphfluidsimulation_amd import scene_json
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "app", "assets", "simulationScenes",
+                        "triangleTestScene.json")
+    tris = scene_json.load_scene(path).triangles
+    v = tris.reshape(-1, 3, 3)
+    rng = np.random.default_rng(3)
+    p = rng.uniform(v.min((0, 1)) - 3, v.max((0, 1)) + 3, size=(300, 3)).astype(np.float32)
+    d = point_triangle_distances(p, tris)
+    checked = 0
+    for i in range(p.shape[0]):
+        for t in np.nonzero(d[i] < 3.0)[0][:50]:
+            f = np.linalg.norm(orc.triangle_force(p[i], tris[t]).astype(np.float64))
+            if d[i, t] > 2.5 * (1 + 1e-6):
+                assert f == 0.0
+            elif 0.05 < d[i, t] < 2.4:
+                tt = 2.5 / d[i, t] - 1.0
+                assert abs(f - 50 * tt * tt) <= 1e-4 * 50 * tt * tt
+                checked += 1
+    assert checked > 500
