@@ -79,16 +79,17 @@ KERNELS = {
 
 def load_traffic(kernel: str, workload: str):
     """HBM bytes per launch of a kernel from a committed rocprofv3 --pmc summary
-    (profiles/<kernel>_<workload>_pmc.md, line 'traffic_bytes: X'), if any."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_pmc.md")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        if line.startswith("traffic_bytes:"):
-            try:
-                return float(line.split(":", 1)[1])
-            except ValueError:
-                return None
+    (profiles/<kernel>_<workload>_pmc.md or _trace.md, line 'traffic_bytes: X'), if any."""
+    for kind in ("pmc", "trace"):
+        path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_{kind}.md")
+        if not os.path.exists(path):
+            continue
+        for line in open(path):
+            if line.startswith("traffic_bytes:"):
+                try:
+                    return float(line.split(":", 1)[1])
+                except ValueError:
+                    return None
     return None
 
 
