@@ -170,7 +170,11 @@ def main():
     ap.add_argument("--mixed", action="store_true",
                     help="mixed precision: FP16-offset candidate filter, Kahan-compensated neighbour sums")
     ap.add_argument("--autotune", action="store_true",
-                    help="after initialize, let the engine time every walk and keep the fastest")
+                    help="after initialize, let the engine time every walk (and the default walk's launch "
+                         "shape) and keep the fastest")
+    ap.add_argument("--shape", default=None, metavar="D,F",
+                    help="launch shape of the default walk: waves per workgroup of the density and the force "
+                         "pass (1, 2, 4 or 8 each)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=10.0,
                     help="seconds of CPU baseline work with all threads (half that again with one thread)")
@@ -228,6 +232,10 @@ def main():
         e = SimulationEngineHIP(0)
         e.Initialize(scene_list if len(scene_list) > 1 else scene_list[0], flags=flags)
         engines.append(e)
+    if args.shape:
+        d, f = (int(x) for x in args.shape.split(","))
+        for e in engines:
+            e.set_launch_shape(d, f)
     if args.autotune:
         chosen = {e.autotune(dt, 4) for e in engines}
         args.walk = "/".join(sorted(chosen))
@@ -235,6 +243,7 @@ def main():
         if handoff:
             args.neighbours = "/".join(sorted(handoff))
         log(f"[bench] autotune chose {args.walk} ({args.neighbours})")
+    shapes = sorted({e.launch_shape() for e in engines})
     n_total = sum(e.n for e in engines)
     M_total = sum(e.hash_map_size(s) for e in engines for s in range(e.n_scenes))
     Ms_total = sum(sc.n_static for sc in scene_list)
@@ -326,6 +335,7 @@ def main():
             "walk": args.walk,
             "neighbours": args.neighbours,
             "autotune": bool(args.autotune),
+            "launch_shape": "/".join(f"density {d}w, force {f}w" for d, f in shapes),
             "precision": "mixed (fp16 filter, kahan sums)" if args.mixed else "fp32",
         },
         "roofline": {

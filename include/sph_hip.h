@@ -269,9 +269,10 @@ int sph_stage_times(sph_handle h, double* ms, uint64_t* launches);
  * LDS bucket tiles) for 2 warm-up steps plus trial_steps (rounded up to an even
  * count, at least 2) steps of dt on the current state, keeps the fastest for
  * later updates (the configured variant unless another is more than 3% faster)
- * and writes its flags to *chosen: the walk flag (0,
- * SPH_FLAG_RUN_WALK or SPH_FLAG_TILE_WALK), with SPH_FLAG_CW_LISTS or
- * SPH_FLAG_CW_MASKS added for the cell-group waves. It allocates whichever of
+ * and, for the cell-group waves, their launch shape the same way (waves per
+ * workgroup of each pass; sph_launch_shape reads it), and writes its flags to
+ * *chosen: the walk flag (0, SPH_FLAG_RUN_WALK or SPH_FLAG_TILE_WALK), with
+ * SPH_FLAG_CW_LISTS or SPH_FLAG_CW_MASKS added for the cell-group waves. It allocates whichever of
  * the lists and masks the engine lacks and keeps both. The state, the
  * simulation time and the results of later updates (within the float tolerance
  * of the variants' summation orders) are as if it had not been called: each
@@ -279,6 +280,17 @@ int sph_stage_times(sph_handle h, double* ms, uint64_t* launches);
  * copies of positions, velocities, ids and the cell-start table. An engine
  * initialized with SPH_FLAG_CELL_WALK keeps that walk. */
 int sph_autotune(sph_handle h, float dt, uint32_t trial_steps, uint32_t* chosen);
+
+/* Launch shape of the default walk's neighbour passes, no counterpart in the
+ * reference: waves per workgroup of the density pass and of the force pass (1, 2,
+ * 4 or 8 waves of 64 lanes, one particle per lane). Every wave keeps its
+ * 64-particle slice, so results do not depend on the shape. Until it is set,
+ * sph_initialize picks 4, 4 over neighbour lists and 2, 8 over masks; a shape
+ * set here is kept by later sph_initialize calls. sph_autotune chooses it with
+ * the walk. Cell size is not a tuning parameter:
+ * keys and bucket tables must equal the reference's, whose cells are h. */
+int sph_set_launch_shape(sph_handle h, uint32_t density_waves, uint32_t force_waves);
+int sph_launch_shape(sph_handle h, uint32_t* density_waves, uint32_t* force_waves);
 
 /* Measurement aid with no counterpart in the reference: launches `repeats` plain
  * copies of `bytes` bytes between two device buffers (kernel k_copy_probe), so

@@ -79,6 +79,8 @@ SIGNATURES = {
     "sph_read_noalloc": (ctypes.c_int, [_P, _F32P, _F32P, _F32P]),
     "sph_write": (ctypes.c_int, [_P, _F32P, _F32P]),
     "sph_set_reorder_interval": (ctypes.c_int, [_P, ctypes.c_float]),
+    "sph_set_launch_shape": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32]),
+    "sph_launch_shape": (ctypes.c_int, [_P, _U32P, _U32P]),
     "sph_read_keys": (ctypes.c_int, [_P, _U32P]),
     "sph_read_cell_starts": (ctypes.c_int, [_P, _U32P]),
     "sph_read_bucket_ids": (ctypes.c_int, [_P, _U32P]),

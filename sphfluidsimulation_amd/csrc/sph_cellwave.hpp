@@ -30,8 +30,9 @@
 
 namespace sph {
 
-constexpr uint32_t kWaves = kBlock / 64;  // waves per workgroup
+constexpr uint32_t kWaves = kBlock / 64;  // 64-particle slices ("waves") of a 256-particle block
 constexpr uint32_t kGroupBatch = 16;      // groups whose bucket tables a wave keeps at once
+constexpr uint32_t kMaxWG = 512;          // largest workgroup of the launch shapes (8 waves)
 constexpr uint32_t kCwMaskWords = 8;      // 4 rounds x 64 bits per particle
 // Candidates per bucket in the round layout: position l = 64 r + lane of round r
 // holds slot l % kSlots of bucket l / kSlots. Nine slots (243 positions) still
@@ -46,11 +47,39 @@ static_assert(kSlots == 8 || kSlots == 9, "27 buckets of kSlots positions must f
 __device__ inline uint32_t slot_bucket(uint32_t l) { return kSlots == 8 ? l >> 3 : (l * 57u) >> 9; }
 
 // Per-wave scratch: bucket start and count of the 27 neighbour buckets of up to
-// kGroupBatch groups.
-struct GroupTables {
-    uint32_t start[kWaves][kGroupBatch][27];
-    uint32_t cnt[kWaves][kGroupBatch][27];
+// kGroupBatch groups. A workgroup of W waves holds W of them in dynamic LDS
+// (wave_tables), so the workgroup size is a launch parameter.
+struct WaveTables {
+    uint32_t start[kGroupBatch][27];
+    uint32_t cnt[kGroupBatch][27];
 };
+
+__device__ inline WaveTables& wave_tables(uint32_t wave) {
+    extern __shared__ __align__(16) unsigned char sph_dyn_lds[];
+    return reinterpret_cast<WaveTables*>(sph_dyn_lds)[wave];
+}
+
+// Launch shape of the cell-wave passes. Workgroups hold W = blockDim.x / 64 waves,
+// launched in XCD-aware order (xcd_tile over the workgroups); wave gw of the launch
+// takes 64-particle slice gw % 4 of 256-particle block gw / 4 (the block in launch
+// order: `order`, if given). Every slice keeps its place (block, slice), so the
+// results do not depend on the shape. The wave index goes through readfirstlane:
+// the compiler then knows it, the block and the scene are wave-uniform and keeps
+// them (and every load through them) in scalar registers. Returns false for waves
+// past the last block.
+struct WaveSlot {
+    uint32_t tile;   // block index in launch order
+    uint32_t slice;  // the block's 64-particle slice this wave takes
+    uint32_t wave;   // the wave's index in its workgroup
+};
+__device__ inline bool wave_slot(uint32_t numBlocks, uint32_t numWG, WaveSlot& ws) {
+    const uint32_t W = blockDim.x >> 6;
+    ws.wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t gw = xcd_tile(blockIdx.x, numWG) * W + ws.wave;
+    ws.tile = gw / kWaves;
+    ws.slice = gw % kWaves;
+    return ws.tile < numBlocks;
+}
 
 __device__ inline uint64_t ballot64(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
@@ -361,10 +390,10 @@ __device__ inline uint32_t wall_visits(uint32_t wword, uint32_t lane, uint32_t g
 // kPoisonBit alone for a poisoned group (fetch_group), which walks no walls.
 // nanLanes: the lanes whose particle has a NaN coordinate.
 template <bool kStatic, class P1, class P2>
-__device__ inline void for_each_group(uint32_t lane, uint32_t wave, bool valid, uint32_t code, uint32_t noff,
+__device__ inline void for_each_group(uint32_t lane, bool valid, uint32_t code, uint32_t noff,
                                       const SceneDev& S, const uint32_t* __restrict__ table,
                                       const uint32_t* __restrict__ wallMap, uint32_t nan0, uint64_t nanLanes,
-                                      uint32_t lo, uint32_t hi, GroupTables& gt, uint32_t& myWalls, P1&& phase1,
+                                      uint32_t lo, uint32_t hi, WaveTables& gt, uint32_t& myWalls, P1&& phase1,
                                       P2&& phase2) {
     const uint32_t prevCode = (uint32_t)__shfl_up((int)code, 1);
     uint64_t heads = ballot64(valid && (lane == 0u || prevCode != code));
@@ -376,8 +405,8 @@ __device__ inline void for_each_group(uint32_t lane, uint32_t wave, bool valid, 
             const uint32_t s0 = cur.tb > lo ? cur.tb : lo, e0 = cur.te < hi ? cur.te : hi;
             const uint32_t start27 = s0, cnt27 = lane < 27u && e0 > s0 ? e0 - s0 : 0u;  // empty past lane 26
             if (lane < 27u) {
-                gt.start[wave][slot][lane] = start27;
-                gt.cnt[wave][slot][lane] = cnt27;
+                gt.start[slot][lane] = start27;
+                gt.cnt[slot][lane] = cnt27;
             }
             const uint32_t crowded = (uint32_t)ballot64(cnt27 > kSlots);  // buckets 0..26
             const uint32_t walls =
@@ -553,7 +582,7 @@ template <bool kStatic, bool kDebug, bool kMixed, bool kList>
 #ifndef SPH_CW_DENSITY_LIST_WAVES
 #define SPH_CW_DENSITY_LIST_WAVES 6
 #endif
-__global__ __launch_bounds__(kBlock, kMixed ? 6 : (kList && kViscLists ? SPH_CW_DENSITY_LIST_WAVES : SPH_CW_DENSITY_WAVES)) void k_density_cw(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
+__global__ __launch_bounds__(kMaxWG, kMixed ? 6 : (kList && kViscLists ? SPH_CW_DENSITY_LIST_WAVES : SPH_CW_DENSITY_WAVES)) void k_density_cw(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
                                                        const SceneDev* __restrict__ scenes, float4* pos,
                                                        const float4* __restrict__ vel,
                                                        const uint32_t* __restrict__ tab,
@@ -563,16 +592,16 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : (kList && kViscLists ? SPH_CW_
                                                        const uint32_t* __restrict__ order,
                                                        uint32_t* __restrict__ nbr,
                                                        const uint32_t* __restrict__ nanCnt,
-                                                       const DensityOut* __restrict__ out) {
-    __shared__ GroupTables gt;
-    const uint32_t tile = xcd_tile(blockIdx.x, numBlocks);
-    if (tile >= numBlocks) return;
-    const uint32_t blk = order[tile];
+                                                       const DensityOut* __restrict__ out, uint32_t numWG) {
+    WaveSlot ws;
+    if (!wave_slot(numBlocks, numWG, ws)) return;  // only wave-level sync below
+    WaveTables& gt = wave_tables(ws.wave);
+    const uint32_t blk = order[ws.tile];
     const BlockDesc bd = blocks[blk];
     const SceneDev& S = scenes[bd.scene];
-    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    const uint32_t lane = __lane_id(), wave = ws.slice;  // the block's 64-particle slice
     const uint32_t wbegin = bd.begin + 64u * wave;
-    if (wbegin >= bd.end) return;  // whole wave leaves; only wave-level sync below
+    if (wbegin >= bd.end) return;  // whole wave leaves
     const uint32_t wcount = min(64u, bd.end - wbegin);
     const uint32_t i = wbegin + lane;
     const bool valid = lane < wcount;
@@ -686,7 +715,7 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : (kList && kViscLists ? SPH_CW_
     uint32_t walls = 0u;  // static buckets this lane walks, or kPoisonBit
     const uint64_t nanLanes = ballot64(valid && (xi.x != xi.x || xi.y != xi.y || xi.z != xi.z));
     for_each_group<kStatic>(
-        lane, wave, valid, code, lane_offsets(lane), S, table, wallMap, nanCnt[bd.scene], nanLanes, lo, hi, gt, walls,
+        lane, valid, code, lane_offsets(lane), S, table, wallMap, nanCnt[bd.scene], nanLanes, lo, hi, gt, walls,
         [&](uint32_t, uint32_t g0, uint32_t g1, uint32_t start27, uint32_t cnt27, auto&& between) {
             // The particle itself passes its own test (r = 0); dn skips it. Empty
             // slots are masked out with the (wave-uniform) slot masks.
@@ -755,7 +784,7 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : (kList && kViscLists ? SPH_CW_
             ViscSums<kMixed> vs;
             if constexpr (kVisc) vs.vi = ld16(vel, i);
             auto dnv = [&](const auto& j) { return dn(j, vs); };
-            walk_mask<kList ? kBatchDensityList : kBatchDensity, false>(m, gt.start[wave][slot], i, dnv);
+            walk_mask<kList ? kBatchDensityList : kBatchDensity, false>(m, gt.start[slot], i, dnv);
             // Crowded buckets: their particles past the kSlots-th, a batch at a time.
             // Mask mode: extraMask[i] bit k says extra particle k (enumerated bucket
             // by bucket in visit order) is kept; all ones when there are more than
@@ -765,7 +794,7 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : (kList && kViscLists ? SPH_CW_
             const bool anyCrowded = crowded != 0u;
             for (; crowded; crowded &= crowded - 1u) {
                 const uint32_t bi = (uint32_t)__builtin_ctz(crowded);
-                const uint32_t s0 = gt.start[wave][slot][bi], e = s0 + gt.cnt[wave][slot][bi];
+                const uint32_t s0 = gt.start[slot][bi], e = s0 + gt.cnt[slot][bi];
                 for (uint32_t jj = s0 + kSlots; jj < e; jj += kBatchCrowded) {
                     uint32_t j[kBatchCrowded];
 #pragma unroll
@@ -970,7 +999,7 @@ __device__ inline void force_finish(ForceSums<kMixed, kVisc>& f, const SceneDev&
 // cell groups, tables or masks. A particle whose list overflowed walks its 27
 // buckets by Morton runs instead.
 template <bool kStatic, bool kDebug, bool kTri, bool kMixed>
-__global__ __launch_bounds__(kBlock) void k_force_cw(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
+__global__ __launch_bounds__(kMaxWG) void k_force_cw(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
                                                      const SceneDev* __restrict__ scenes,
                                                      const float4* __restrict__ pos, const float4* __restrict__ vel,
                                                      const uint32_t* __restrict__ tab,
@@ -982,13 +1011,15 @@ __global__ __launch_bounds__(kBlock) void k_force_cw(const BlockDesc* __restrict
                                                      uint32_t* __restrict__ nanNext,
                                                      const uint32_t* __restrict__ ids, const float* __restrict__ tris,
                                                      float dt, float4* __restrict__ posT, float4* __restrict__ velT,
-                                                     uint32_t* __restrict__ counts, float4* __restrict__ accOut) {
-    const uint32_t blk = xcd_tile(blockIdx.x, numBlocks);
-    if (blk >= numBlocks) return;
+                                                     uint32_t* __restrict__ counts, float4* __restrict__ accOut,
+                                                     uint32_t numWG) {
+    WaveSlot ws;
+    if (!wave_slot(numBlocks, numWG, ws)) return;
+    const uint32_t blk = ws.tile;
     const BlockDesc bd = blocks[blk];
     const SceneDev& S = scenes[bd.scene];
-    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
-    const uint32_t i = bd.begin + threadIdx.x;
+    const uint32_t lane = __lane_id(), wave = ws.slice;
+    const uint32_t i = bd.begin + 64u * wave + lane;
     if (i >= bd.end) return;  // the active lanes of a wave stay a prefix
     ForceSums<kMixed, kViscLists> f;  // the density pass holds the viscosity sums
     f.xi = ld16(pos, i);
@@ -1020,8 +1051,8 @@ __global__ __launch_bounds__(kBlock) void k_force_cw(const BlockDesc* __restrict
                   });
     }
     f.take_visc(visc);
-    force_finish<kStatic, kDebug, kTri, kMixed>(f, S, bd.scene, lane, n == kListPoison, wallSum, wallZ, ids, tris,
-                                                dt, posT, velT, counts, accOut, nanNext);
+    force_finish<kStatic, kDebug, kTri, kMixed>(f, S, bd.scene, lane, n == kListPoison, wallSum, wallZ, ids, tris, dt,
+                                                posT, velT, counts, accOut, nanNext);
 }
 
 // Force pass over the density pass's neighbour masks, for scenes too large for
@@ -1031,20 +1062,21 @@ __global__ __launch_bounds__(kBlock) void k_force_cw(const BlockDesc* __restrict
 // Eight waves per SIMD (at most 64 VGPRs) in FP32; the mixed mode's Kahan sums
 // need more registers, so that mode settles for six.
 template <bool kStatic, bool kDebug, bool kTri, bool kMixed>
-__global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw_mask(
+__global__ __launch_bounds__(kMaxWG, kMixed ? 6 : 8) void k_force_cw_mask(
     const BlockDesc* __restrict__ blocks, uint32_t numBlocks, const SceneDev* __restrict__ scenes,
     const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ tab,
     const float4* __restrict__ wallSum, const float4* __restrict__ wallZ, const float4* __restrict__ visc,
     const uint32_t* __restrict__ mask,
     const uint64_t* __restrict__ extraMask, const uint32_t* __restrict__ nanCnt, uint32_t* __restrict__ nanNext,
     const uint32_t* __restrict__ ids, const float* __restrict__ tris, float dt, float4* __restrict__ posT,
-    float4* __restrict__ velT, uint32_t* __restrict__ counts, float4* __restrict__ accOut) {
-    __shared__ GroupTables gt;
-    const uint32_t blk = xcd_tile(blockIdx.x, numBlocks);
-    if (blk >= numBlocks) return;
+    float4* __restrict__ velT, uint32_t* __restrict__ counts, float4* __restrict__ accOut, uint32_t numWG) {
+    WaveSlot ws;
+    if (!wave_slot(numBlocks, numWG, ws)) return;
+    WaveTables& gt = wave_tables(ws.wave);
+    const uint32_t blk = ws.tile;
     const BlockDesc bd = blocks[blk];
     const SceneDev& S = scenes[bd.scene];
-    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    const uint32_t lane = __lane_id(), wave = ws.slice;
     const uint32_t wbegin = bd.begin + 64u * wave;
     if (wbegin >= bd.end) return;
     const uint32_t wcount = min(64u, bd.end - wbegin);
@@ -1071,11 +1103,11 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw_mask(
     uint32_t noWalls = 0u;  // walls come from the density pass's sums; kPoisonBit
     const uint64_t nanLanes = ballot64(valid && (f.xi.x != f.xi.x || f.xi.y != f.xi.y || f.xi.z != f.xi.z));
     for_each_group<false>(
-        lane, wave, valid, code, lane_offsets(lane), S, tab + S.tBegin, nullptr, nanCnt[bd.scene], nanLanes,
+        lane, valid, code, lane_offsets(lane), S, tab + S.tBegin, nullptr, nanCnt[bd.scene], nanLanes,
         S.pBegin, S.pBegin + S.pCount, gt, noWalls,
         [&](uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, auto&& between) { between(); },
         [&](uint32_t slot, uint32_t crowded) {
-            walk_mask<kBatchForceMask, true>(m, gt.start[wave][slot], i, pairs);
+            walk_mask<kBatchForceMask, true>(m, gt.start[slot], i, pairs);
             // Crowded buckets' extra particles: the density pass's neighbours
             // among them when there are at most kExtraBits, else every one.
             if (!crowded) return;
@@ -1083,7 +1115,7 @@ __global__ __launch_bounds__(kBlock, kMixed ? 6 : 8) void k_force_cw_mask(
             const bool all = xb == ~0ull;
             for (; crowded; crowded &= crowded - 1u) {
                 const uint32_t bi = (uint32_t)__builtin_ctz(crowded);
-                const uint32_t s0 = gt.start[wave][slot][bi] + kSlots, ne = gt.cnt[wave][slot][bi] - kSlots;
+                const uint32_t s0 = gt.start[slot][bi] + kSlots, ne = gt.cnt[slot][bi] - kSlots;
                 for (uint32_t base = 0; base < ne; base += 32u) {
                     const uint32_t c = min(32u, ne - base);
                     const uint32_t low = c < 32u ? (1u << c) - 1u : ~0u;

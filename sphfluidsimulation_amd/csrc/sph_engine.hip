@@ -90,6 +90,10 @@ struct sph_engine {
     uint32_t* nbrCount = nullptr;   // cell-wave: list lengths
     DensityOut* densOut = nullptr;  // cell-wave density pass: its output pointers (device copy)
     uint32_t* nanCnt = nullptr;     // 2 x nScenes: NaN particles in bucket 0, per parity
+    // Launch shape of the cell-wave passes: waves per workgroup of the density and
+    // force passes; shapeSet once the caller chose it (sph_set_launch_shape).
+    uint32_t densWaves = 4, forceWaves = 4;
+    bool shapeSet = false;
     double* consPartial = nullptr;
     double* consOut = nullptr;
     uint32_t* valBits = nullptr;
@@ -397,6 +401,12 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     const uint32_t prof = capture ? 0u : h->profiling;
     const dim3 grid(h->numBlocks), block(kBlock);
     const dim3 gridX(((h->numBlocks + 7u) / 8u) * 8u);  // padded for the XCD-aware tile order
+    // Cell-wave passes: workgroups of W waves, each wave a 64-particle slice of a block.
+    auto wgs = [&](uint32_t W) { return (h->numBlocks * kWaves + W - 1u) / W; };
+    const uint32_t densWG = wgs(h->densWaves), forceWG = wgs(h->forceWaves);
+    const dim3 gridD(((densWG + 7u) / 8u) * 8u), blockD(64u * h->densWaves);
+    const dim3 gridF(((forceWG + 7u) / 8u) * 8u), blockF(64u * h->forceWaves);
+    const size_t ldsD = sizeof(WaveTables) * h->densWaves, ldsF = sizeof(WaveTables) * h->forceWaves;
     StageEvent ev{};
     bool timing = false;
     auto begin = [&](int stage) -> int {
@@ -428,10 +438,9 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     uint32_t* nanNext = h->nanCnt + (size_t)(c ^ 1) * h->nScenes;
     if (walk != Walk::Cells) {
 #define SPH_DENS_CW(S, D, X, L)                                                                                \
-        hipLaunchKernelGGL((k_density_cw<S, D, X, L>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,   \
-                           h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->wallMap, h->tileOrder,  \
-                           h->nbr,                                                                             \
-                           nanNow, h->densOut)
+        hipLaunchKernelGGL((k_density_cw<S, D, X, L>), gridD, blockD, ldsD, h->stream, h->dBlocks,            \
+                           h->numBlocks, h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->wallMap,  \
+                           h->tileOrder, h->nbr, nanNow, h->densOut, densWG)
 #define SPH_DENS(S, D)                                                                                         \
     if (walk == Walk::CellWave && mixed && h->cwList)                                                          \
         SPH_DENS_CW(S, D, true, true);                                                                         \
@@ -488,15 +497,15 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     // Forces, integration, new keys, histogram into the other table.
     if ((rc = begin(SPH_STAGE_FORCE))) return rc;
 #define SPH_FORCE_LIST(S, D, T, X)                                                                                \
-        hipLaunchKernelGGL((k_force_cw<S, D, T, X>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,       \
+        hipLaunchKernelGGL((k_force_cw<S, D, T, X>), gridF, blockF, 0, h->stream, h->dBlocks, h->numBlocks,      \
                            h->dScenes, h->pos, h->vel, h->tab[c], h->wallSum, h->wallZ, h->visc, h->nbr,          \
-                           h->nbrCount,                                                                           \
-                           nanNext, h->ids[c], h->tris, dt, h->posT, h->velT, h->tab[c ^ 1], h->acc)
+                           h->nbrCount, nanNext, h->ids[c], h->tris, dt, h->posT, h->velT, h->tab[c ^ 1], h->acc, \
+                           forceWG)
 #define SPH_FORCE_MASK(S, D, T, X)                                                                                \
-        hipLaunchKernelGGL((k_force_cw_mask<S, D, T, X>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,  \
-                           h->dScenes, h->pos, h->vel, h->tab[c], h->wallSum, h->wallZ, h->visc, h->mask,         \
-                           h->extraMask,                                                                          \
-                           nanNow, nanNext, h->ids[c], h->tris, dt, h->posT, h->velT, h->tab[c ^ 1], h->acc)
+        hipLaunchKernelGGL((k_force_cw_mask<S, D, T, X>), gridF, blockF, ldsF, h->stream, h->dBlocks,            \
+                           h->numBlocks, h->dScenes, h->pos, h->vel, h->tab[c], h->wallSum, h->wallZ, h->visc,    \
+                           h->mask, h->extraMask, nanNow, nanNext, h->ids[c], h->tris, dt, h->posT, h->velT,      \
+                           h->tab[c ^ 1], h->acc, forceWG)
 #define SPH_FORCE(S, D, T)                                                                                        \
     do {                                                                                                          \
     if (walk == Walk::CellWave && mixed && h->cwList)                                                             \
@@ -865,6 +874,13 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
         if (kViscLists) SPH_ALLOC(h->visc, std::max<uint64_t>(N, 1));
         // Lists while they stay in the caches (4 B per neighbour), else masks (32 B per particle).
         h->cwList = (flags & SPH_FLAG_CW_LISTS) || (!(flags & SPH_FLAG_CW_MASKS) && N <= SPH_CW_LIST_MAX);
+        if (!h->shapeSet) {
+            // Measured on MI355X: over lists 4 + 4 waves per workgroup; over masks the
+            // force pass (which rebuilds the cell groups) gains 3% from 8 waves, the
+            // density pass 1% from 2 (dam-break-8m).
+            h->densWaves = h->cwList ? 4u : 2u;
+            h->forceWaves = h->cwList ? 4u : 8u;
+        }
         if (h->cwList) {
             SPH_ALLOC(h->nbr, (uint64_t)std::max<uint32_t>(h->numBlocks, 1) * kWaves * kListCap * 64u);
             SPH_ALLOC(h->nbrCount, std::max<uint64_t>(N, 1));
@@ -1435,6 +1451,7 @@ int sph_autotune(sph_handle h, float dt, uint32_t trial_steps, uint32_t* chosen)
     const uint32_t savedProfiling = h->profiling;
     const uint32_t savedFlags = h->flags;
     const bool savedList = h->cwList;
+    const uint32_t savedShape[2] = {h->densWaves, h->forceWaves};
     h->profiling = 0;
     float4 *bPos = nullptr, *bVel = nullptr;
     uint32_t *bIds = nullptr, *bTab = nullptr, *bNan = nullptr;
@@ -1482,6 +1499,39 @@ int sph_autotune(sph_handle h, float dt, uint32_t trial_steps, uint32_t* chosen)
         h->err = std::string("sph_autotune: ") + hipGetErrorString(e);
         return finish(SPH_E_HIP);
     }
+    // A trial that fails leaves the engine as configured: the starting parity, the
+    // configured variant and shape, and the state of the snapshot.
+    auto abandon = [&](int code) {
+        h->cur = c;
+        h->flags = savedFlags;
+        h->cwList = savedList;
+        h->densWaves = savedShape[0];
+        h->forceWaves = savedShape[1];
+        (void)hipStreamSynchronize(h->stream);
+        (void)restore();
+        (void)hipStreamSynchronize(h->stream);
+        return finish(code);
+    };
+    // One trial of the engine's current variant and shape from the snapshot: warm-up
+    // steps, then the timed ones; the state is restored afterwards.
+    auto trial = [&](float& ms) -> int {
+        for (uint32_t s = 0; s < warm + timed; ++s) {
+            if (s == warm && (e = hipEventRecord(e0, h->stream)) != hipSuccess) break;
+            if ((rc = enqueue_step(h, dt, false))) return rc;
+            h->cur ^= 1;
+        }
+        if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        h->cur = c;
+        if (e == hipSuccess) e = restore();
+        if (e != hipSuccess) {
+            h->err = std::string("sph_autotune: ") + hipGetErrorString(e);
+            return SPH_E_HIP;
+        }
+        return SPH_OK;
+    };
     // The engine's configured variant first, then the others among the cell-wave
     // walk with lists and with masks, the Morton runs and the bucket tiles. A
     // variant replaces the best so far only when it is more than 3% faster: the
@@ -1500,36 +1550,12 @@ int sph_autotune(sph_handle h, float dt, uint32_t trial_steps, uint32_t* chosen)
     bool bestLists = h->cwList;
     float bestMs = 0.f;
     bool first = true;
-    // A trial that fails to enqueue leaves the engine as configured: the
-    // starting parity, the configured variant and the state of the snapshot.
-    auto abandon = [&](int code) {
-        h->cur = c;
-        h->flags = savedFlags;
-        h->cwList = savedList;
-        (void)hipStreamSynchronize(h->stream);
-        (void)restore();
-        (void)hipStreamSynchronize(h->stream);
-        return finish(code);
-    };
     for (int ci = 0; ci < 4; ++ci) {
         const uint32_t w = candidates[ci];
         h->flags = base | w;
         if (w == 0u) h->cwList = candLists[ci];
-        for (uint32_t s = 0; s < warm + timed; ++s) {
-            if (s == warm && (e = hipEventRecord(e0, h->stream)) != hipSuccess) break;
-            if ((rc = enqueue_step(h, dt, false))) return abandon(rc);
-            h->cur ^= 1;
-        }
-        if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
-        if (e == hipSuccess) e = hipEventSynchronize(e1);
         float ms = 0.f;
-        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-        h->cur = c;
-        if (e == hipSuccess) e = restore();
-        if (e != hipSuccess) {
-            h->err = std::string("sph_autotune: ") + hipGetErrorString(e);
-            return abandon(SPH_E_HIP);
-        }
+        if ((rc = trial(ms))) return abandon(rc);
         if (first || ms < 0.97f * bestMs) {
             best = w;
             if (w == 0u) bestLists = candLists[ci];
@@ -1539,7 +1565,30 @@ int sph_autotune(sph_handle h, float dt, uint32_t trial_steps, uint32_t* chosen)
     }
     h->cwList = bestLists;
     h->flags = base | best | handoff(best, bestLists);
-    for (auto& g : h->graph)  // captured steps may hold another walk's kernels
+    // The cell-wave walk's launch shape, when that walk won: waves per workgroup of
+    // the density pass, then of the force pass; each replaces the configured one
+    // only when more than 3% faster. The results are the same whatever the shape.
+    if (best == 0u) {
+        float curMs = 0.f;
+        if ((rc = trial(curMs))) return abandon(rc);
+        const uint32_t waves[4] = {1u, 2u, 4u, 8u};
+        for (int pass = 0; pass < 2; ++pass) {
+            uint32_t& field = pass == 0 ? h->densWaves : h->forceWaves;
+            uint32_t bestW = field;
+            for (uint32_t w : waves) {
+                if (w == savedShape[pass]) continue;
+                field = w;
+                float ms = 0.f;
+                if ((rc = trial(ms))) return abandon(rc);
+                if (ms < 0.97f * curMs) {
+                    curMs = ms;
+                    bestW = w;
+                }
+            }
+            field = bestW;
+        }
+    }
+    for (auto& g : h->graph)  // captured steps may hold another walk's kernels or shape
         if (g) {
             (void)hipGraphExecDestroy(g);
             g = nullptr;
@@ -1552,6 +1601,33 @@ int sph_autotune(sph_handle h, float dt, uint32_t trial_steps, uint32_t* chosen)
     }
     if (chosen) *chosen = best | handoff(best, bestLists);
     return finish(SPH_OK);
+}
+
+int sph_set_launch_shape(sph_handle h, uint32_t density_waves, uint32_t force_waves) {
+    if (!valid(h)) return SPH_E_INVALID;
+    auto wavesOk = [](uint32_t w) { return w == 1u || w == 2u || w == 4u || w == 8u; };
+    if (!wavesOk(density_waves) || !wavesOk(force_waves)) {
+        h->err = "sph_set_launch_shape: waves per workgroup must be 1, 2, 4 or 8";
+        return SPH_E_INVALID;
+    }
+    SPH_HIP(h, hipSetDevice(h->device));
+    SPH_HIP(h, hipStreamSynchronize(h->stream));
+    h->densWaves = density_waves;
+    h->forceWaves = force_waves;
+    h->shapeSet = true;
+    for (auto& g : h->graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    return SPH_OK;
+}
+
+int sph_launch_shape(sph_handle h, uint32_t* density_waves, uint32_t* force_waves) {
+    if (!valid(h)) return SPH_E_INVALID;
+    if (density_waves) *density_waves = h->densWaves;
+    if (force_waves) *force_waves = h->forceWaves;
+    return SPH_OK;
 }
 
 int sph_copy_probe(sph_handle h, uint64_t bytes, uint32_t repeats) {

@@ -254,6 +254,17 @@ class SimulationEngineHIP:
                            "masks" if c & _lib.SPH_FLAG_CW_MASKS else None)
         return self.WALK_NAMES[walk]
 
+    def set_launch_shape(self, density_waves: int = 4, force_waves: int = 4):
+        """Waves per workgroup (1, 2, 4 or 8) of the default walk's density and
+        force passes (results unchanged)."""
+        self._check(self._lib.sph_set_launch_shape(self._h, int(density_waves), int(force_waves)))
+
+    def launch_shape(self):
+        """(density waves, force waves) per workgroup."""
+        v = [ctypes.c_uint32(0) for _ in range(2)]
+        self._check(self._lib.sph_launch_shape(self._h, *(ctypes.byref(x) for x in v)))
+        return tuple(int(x.value) for x in v)
+
     # -- asynchronous readback (RenderableGPUParticleBufferManager.cpp:190-270) --
     def readback_enqueue(self) -> int:
         """Enqueue a copy of the state after the steps enqueued so far into the

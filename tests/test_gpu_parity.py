@@ -531,6 +531,61 @@ def test_particle_map_mode_batched():
     e.close()
 
 
+SHAPES = [(4, 4), (1, 1), (2, 8), (8, 2), (1, 4)]
+
+
+@pytest.mark.parametrize("mode", [pytest.param(0, id="lists"), pytest.param(SPH_FLAG_CW_MASKS, id="masks")])
+@pytest.mark.parametrize("batched", [False, True])
+def test_launch_shapes_give_identical_results(s16k, mode, batched):
+    """Every launch shape of the default walk (waves per workgroup of each pass)
+    computes each particle from the same 64-particle slice and sums in the same
+    order, so one step from the same state is bit-identical to the default
+    shape's."""
+    scs = scenes.batch_scenes(3, 2000, 9000, seed=37) if batched else s16k
+    say(f"launch shapes {SHAPES} ({'batch of 3' if batched else '16k'}, handoff flags {mode})")
+    want = None
+    for shape in SHAPES:
+        e = SimulationEngineHIP(0)
+        e.Initialize(scs, flags=SPH_FLAG_RECORD_DEBUG | mode)
+        e.set_launch_shape(*shape)
+        assert e.launch_shape() == shape
+        e.Update(1e-3, 1)
+        got = (*e.read_debug(), *e.read(), e.read_keys(), *e.read_cell_starts())
+        e.close()
+        if want is None:
+            want = got
+            continue
+        for g, w in zip(got, want):
+            np.testing.assert_array_equal(g, w)
+
+
+def test_default_launch_shape_follows_the_handoff(s16k):
+    """4 + 4 waves per workgroup over lists, 2 + 8 over masks, until the caller
+    sets a shape, which later initializations keep."""
+    e = SimulationEngineHIP(0)
+    e.Initialize(s16k)
+    assert e.launch_shape() == (4, 4)
+    e.Initialize(s16k, flags=SPH_FLAG_CW_MASKS)
+    assert e.launch_shape() == (2, 8)
+    e.set_launch_shape(1, 2)
+    e.Initialize(s16k)
+    assert e.launch_shape() == (1, 2)
+    e.close()
+
+
+def test_autotune_picks_a_launch_shape(s16k):
+    e = SimulationEngineHIP(0)
+    e.Initialize(s16k)
+    e.Update(1e-3, 2)
+    walk = e.autotune(1e-3, 4)
+    d, f = e.launch_shape()
+    say(f"autotune: {walk} ({e.neighbours}), shape {d, f}")
+    assert d in (1, 2, 4, 8) and f in (1, 2, 4, 8)
+    e.Update(1e-3, 2)
+    assert e.validate() == 0
+    e.close()
+
+
 def test_autotune_leaves_state_unchanged(s16k):
     say("autotune: every walk timed from a copy of the state")
     e = SimulationEngineHIP(0)
