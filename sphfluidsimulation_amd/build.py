@@ -99,12 +99,23 @@ def build_walk_check():
     return WALK_BIN
 
 
+WALL_LISTS_SRC = os.path.join(ROOT, "tests", "cpp", "wall_lists_check.cpp")
+WALL_LISTS_BIN = os.path.join(ROOT, "tests", "cpp", "wall_lists_check")
+
+
+def build_wall_lists_check():
+    """Host-side check of the density pass's wall lists (compiled by hipcc, runs on the CPU)."""
+    _run([HIPCC, "-O2", "-std=c++17", "-o", WALL_LISTS_BIN, WALL_LISTS_SRC])
+    return WALL_LISTS_BIN
+
+
 def build_all():
     build_hip()
     build_oracle()
     build_adapter_demo()
     build_reference_binding()
     build_walk_check()
+    build_wall_lists_check()
 
 
 if __name__ == "__main__":

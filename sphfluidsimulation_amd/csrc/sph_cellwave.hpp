@@ -27,6 +27,7 @@
 #pragma once
 
 #include "sph_kernels.hpp"
+#include "sph_wall_lists.hpp"
 
 namespace sph {
 
@@ -388,8 +389,10 @@ __device__ inline uint32_t wall_visits(uint32_t wword, uint32_t lane, uint32_t g
 // of its buckets holding more than kSlots particles. myWalls receives, on every
 // lane of a group, the static buckets its particles walk (wall_visits), or
 // kPoisonBit alone for a poisoned group (fetch_group), which walks no walls.
+// With kWallLists the caller walks the wall lists instead, and myWalls is set
+// (to all 27 cells' buckets) only for a group holding a NaN particle.
 // nanLanes: the lanes whose particle has a NaN coordinate.
-template <bool kStatic, class P1, class P2>
+template <bool kStatic, bool kWallLists, class P1, class P2>
 __device__ inline void for_each_group(uint32_t lane, bool valid, uint32_t code, uint32_t noff,
                                       const SceneDev& S, const uint32_t* __restrict__ table,
                                       const uint32_t* __restrict__ wallMap, uint32_t nan0, uint64_t nanLanes,
@@ -400,7 +403,8 @@ __device__ inline void for_each_group(uint32_t lane, bool valid, uint32_t code, 
     const uint64_t stops = heads | ~ballot64(valid);  // a group ends at the next head or invalid lane
     while (heads) {
         uint32_t mySlot = 0xFFFFFFFFu, myCrowded = 0u;
-        GroupFetch cur = fetch_group<kStatic>(heads, stops, lane, code, noff, S, table, wallMap, nan0, nanLanes);
+        GroupFetch cur =
+            fetch_group<kStatic && !kWallLists>(heads, stops, lane, code, noff, S, table, wallMap, nan0, nanLanes);
         for (uint32_t slot = 0;; ++slot) {
             const uint32_t s0 = cur.tb > lo ? cur.tb : lo, e0 = cur.te < hi ? cur.te : hi;
             const uint32_t start27 = s0, cnt27 = lane < 27u && e0 > s0 ? e0 - s0 : 0u;  // empty past lane 26
@@ -409,8 +413,9 @@ __device__ inline void for_each_group(uint32_t lane, bool valid, uint32_t code, 
                 gt.cnt[slot][lane] = cnt27;
             }
             const uint32_t crowded = (uint32_t)ballot64(cnt27 > kSlots);  // buckets 0..26
-            const uint32_t walls =
-                kStatic && !cur.poison ? wall_visits(cur.wword, lane, cur.g0, code, noff, S, cur.nanGroup) : 0u;
+            const uint32_t walls = kStatic && !cur.poison && (!kWallLists || cur.nanGroup)
+                                       ? wall_visits(cur.wword, lane, cur.g0, code, noff, S, cur.nanGroup)
+                                       : 0u;
             if (lane >= cur.g0 && lane < cur.g1) {
                 mySlot = slot;
                 myCrowded = crowded;
@@ -421,7 +426,8 @@ __device__ inline void for_each_group(uint32_t lane, bool valid, uint32_t code, 
             uint64_t rest = more ? heads : 1ull;  // at the batch end: a harmless read of group 0's table
             phase1(slot, cur.g0, cur.g1, start27, cnt27,
                    [&]() {
-                       nxt = fetch_group<kStatic>(rest, stops, lane, code, noff, S, table, wallMap, nan0, nanLanes);
+                       nxt = fetch_group<kStatic && !kWallLists>(rest, stops, lane, code, noff, S, table, wallMap,
+                                                                 nan0, nanLanes);
                    });
             if (!more) break;
             heads = rest;
@@ -547,6 +553,8 @@ struct DensityOut {
     float4* wallZ;        // coincident walls' w1
     float* dens;          // density (debug recording)
     float4* visc;         // viscosity sums over the dynamic neighbours (list mode)
+    const uint32_t* wallCells;  // wall lists (sph_wall_lists.hpp): cell tables
+    const uint32_t* wallList;   // and octant lists
 };
 
 // List mode: the density pass, which visits every kept neighbour, also sums the
@@ -686,18 +694,19 @@ __global__ __launch_bounds__(kMaxWG, kMixed ? 6 : (kList && kViscLists ? SPH_CW_
         return hits;
     };
 
-    // Static neighbours (SF:455-496).
-    auto dWalls = [&](uint32_t j0, uint32_t e) {
+    // Static neighbours (SF:455-496): a batch of kBatchWall walls j, of which the
+    // first `count` are real (the rest repeat one of them).
+    auto dWallBatch = [&](const uint32_t (&j)[kBatchWall], uint32_t count) {
         float4 xj[kBatchWall];
 #pragma unroll
-        for (uint32_t u = 0; u < kBatchWall; ++u) xj[u] = ld16(spos, j0 + u < e ? j0 + u : j0);
+        for (uint32_t u = 0; u < kBatchWall; ++u) xj[u] = ld16(spos, j[u]);
 #pragma unroll
         for (uint32_t u = 0; u < kBatchWall; ++u) {
             const float dx = xj[u].x - xi.x, dy = xj[u].y - xi.y, dz = xj[u].z - xi.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
             const float r = __builtin_amdgcn_sqrtf(r2);
             const float q = h - r;
-            if (j0 + u < e && within_h2(r2, h2)) {
+            if (u < count && within_h2(r2, h2)) {
                 const bool in = inside_h(r, h);
                 sumSt.add(in ? q * q * q : 0.f);
                 const bool zero = r == 0.f;  // r2 == 0 implies r == 0
@@ -712,9 +721,16 @@ __global__ __launch_bounds__(kMaxWG, kMixed ? 6 : (kList && kViscLists ? SPH_CW_
             }
         }
     };
-    uint32_t walls = 0u;  // static buckets this lane walks, or kPoisonBit
+    // A static bucket's walls [j0, e) (walk_walls).
+    auto dWalls = [&](uint32_t j0, uint32_t e) {
+        uint32_t j[kBatchWall];
+#pragma unroll
+        for (uint32_t u = 0; u < kBatchWall; ++u) j[u] = j0 + u < e ? j0 + u : j0;
+        dWallBatch(j, e - j0);
+    };
+    uint32_t walls = 0u;  // static buckets this lane walks (groups with a NaN particle), or kPoisonBit
     const uint64_t nanLanes = ballot64(valid && (xi.x != xi.x || xi.y != xi.y || xi.z != xi.z));
-    for_each_group<kStatic>(
+    for_each_group<kStatic, true>(
         lane, valid, code, lane_offsets(lane), S, table, wallMap, nanCnt[bd.scene], nanLanes, lo, hi, gt, walls,
         [&](uint32_t, uint32_t g0, uint32_t g1, uint32_t start27, uint32_t cnt27, auto&& between) {
             // The particle itself passes its own test (r = 0); dn skips it. Empty
@@ -810,7 +826,13 @@ __global__ __launch_bounds__(kMaxWG, kMixed ? 6 : (kList && kViscLists ? SPH_CW_
             if (!kList && anyCrowded) out->extraMask[i] = k > kExtraBits ? ~0ull : bits;
             if constexpr (kVisc) out->visc[i] = make_float4(vs.x.s, vs.y.s, vs.z.s, 0.f);
         });
-    if (kStatic && ballot64(valid && (walls & ~kPoisonBit) != 0u) != 0ull && lane == 0u) out->heavy[blk] = 1u;
+    // The walls: the lane's wall list, or (a group holding a NaN particle) the
+    // static buckets of all 27 neighbour cells; none for a poisoned particle.
+    uint32_t wl0 = 0u, wl1 = 0u;
+    if (kStatic && valid && walls == 0u)
+        find_wall_list(out->wallCells, S, cx, cy, cz, wall_octant(xi.x, xi.y, xi.z, cx, cy, cz, h), wl0, wl1);
+    if (kStatic && ballot64(valid && ((walls & ~kPoisonBit) != 0u || wl1 > wl0)) != 0ull && lane == 0u)
+        out->heavy[blk] = 1u;
     if (!valid) return;
     if (kList) {
         out->nbrCount[i] = (walls & kPoisonBit) ? kListPoison : nc;
@@ -819,7 +841,16 @@ __global__ __launch_bounds__(kMaxWG, kMixed ? 6 : (kList && kViscLists ? SPH_CW_
         mrow[0] = make_uint4(m8.w[0], m8.w[1], m8.w[2], m8.w[3]);
         mrow[1] = make_uint4(m8.w[4], m8.w[5], m8.w[6], m8.w[7]);
     }
-    if (kStatic) walk_walls<kBatchWall>(walls & ~kPoisonBit, cx, cy, cz, S, stable, dWalls);
+    if (kStatic) {
+        walk_walls<kBatchWall>(walls & ~kPoisonBit, cx, cy, cz, S, stable, dWalls);
+        const uint32_t* __restrict__ wallList = out->wallList;
+        for (uint32_t k = wl0; k < wl1; k += kBatchWall) {
+            uint32_t j[kBatchWall];
+#pragma unroll
+            for (uint32_t u = 0; u < kBatchWall; ++u) j[u] = wallList[k + u < wl1 ? k + u : k];
+            dWallBatch(j, wl1 - k);
+        }
+    }
     // A poisoned particle's density is NaN, as the pile it skipped makes it.
     const float rho = (walls & kPoisonBit) ? __builtin_nanf("")
                                            : S.selfDensity + (sumDyn.s * S.mass + sumSt.s * S.mass) * S.C;
@@ -1102,7 +1133,7 @@ __global__ __launch_bounds__(kMaxWG, kMixed ? 6 : 8) void k_force_cw_mask(
     auto pairs = [&](const auto& j) { f.batch(pos, vel, j); };
     uint32_t noWalls = 0u;  // walls come from the density pass's sums; kPoisonBit
     const uint64_t nanLanes = ballot64(valid && (f.xi.x != f.xi.x || f.xi.y != f.xi.y || f.xi.z != f.xi.z));
-    for_each_group<false>(
+    for_each_group<false, false>(
         lane, valid, code, lane_offsets(lane), S, tab + S.tBegin, nullptr, nanCnt[bd.scene], nanLanes,
         S.pBegin, S.pBegin + S.pCount, gt, noWalls,
         [&](uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, auto&& between) { between(); },

@@ -21,6 +21,7 @@
 #include "sph_kernels.hpp"
 #include "sph_tile.hpp"
 #include "sph_cellwave.hpp"
+#include "sph_wall_lists.hpp"
 
 using namespace sph;
 
@@ -73,11 +74,13 @@ struct sph_engine {
     float4* spos = nullptr;
     uint32_t* stab = nullptr;
     uint32_t* wallMap = nullptr;  // per scene: bit wall_hash(Morton code of c) set for every cell c holding walls
+    uint32_t* wallCells = nullptr;  // wall lists (sph_wall_lists.hpp): per-scene cell tables
+    uint32_t* wallList = nullptr;   // and the octants' lists of static particle indices
     uint32_t* tileOrder = nullptr;  // cell-wave launch order (tile_order_chunk), numBlocks entries
     uint32_t* heavy = nullptr;      // per block: particles near walls in the last density pass
     float4* wallSum = nullptr;      // cell-wave: the walls' geometric sums of each particle, for the force pass
-    float4* wallZ = nullptr;
-    float4* visc = nullptr;         // cell-wave list mode: the density pass's viscosity sums        // and the coincident-pair weight where one occurred
+    float4* wallZ = nullptr;        // and the coincident-pair weight where one occurred
+    float4* visc = nullptr;         // cell-wave list mode: the density pass's viscosity sums
     uint64_t wallWords = 0;
     uint32_t* staticScene = nullptr;
     float* tris = nullptr;
@@ -189,6 +192,8 @@ void release(sph_engine* h) {
     dfree(h->spos);
     dfree(h->stab);
     dfree(h->wallMap);
+    dfree(h->wallCells);
+    dfree(h->wallList);
     dfree(h->tileOrder);
     dfree(h->heavy);
     dfree(h->wallSum);
@@ -728,6 +733,7 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
 
     std::vector<float4> hPos(N), hVel(N), hSpos(Ns);
     std::vector<uint32_t> hIds(N), hOrig(N), hTab(L), hStab, hStaticScene(Ns), hWallMap, hNan(n_scenes, 0u);
+    std::vector<uint32_t> hWallCells, hWallList;
     h->hStaticIds.assign(Ns, 0);
     h->hScenes.resize(n_scenes);
     h->hSceneBlocks.assign(n_scenes + 1, 0);
@@ -788,6 +794,7 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
                 const uint32_t hc = wall_hash(morton3(cell_coord(q[0], S.h), cell_coord(q[1], S.h), cell_coord(q[2], S.h)));
                 hWallMap[S.wmBegin + ((hc >> 5) & S.wmMask)] |= 1u << (hc & 31u);
             }
+            build_wall_lists(S, hSpos.data(), hStab.data() + S.stBegin, hWallCells, hWallList);
         }
         // Dynamic: M = 2N (SimulationEngineCPU.cpp:486-487), one-thread order.
         if (pn) {
@@ -854,6 +861,8 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
     SPH_ALLOC(h->stab, std::max<uint64_t>(h->Ls, 1));
     h->wallWords = hWallMap.size();
     SPH_ALLOC(h->wallMap, std::max<uint64_t>(h->wallWords, 1));
+    SPH_ALLOC(h->wallCells, std::max<uint64_t>(hWallCells.size(), 1));
+    SPH_ALLOC(h->wallList, std::max<uint64_t>(hWallList.size(), 1));
     SPH_ALLOC(h->tileOrder, std::max<uint64_t>(h->numBlocks, 1));
     SPH_ALLOC(h->heavy, std::max<uint64_t>(h->numBlocks, 1));
     if (Ns) {
@@ -922,7 +931,8 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
         SPH_HIP(h, hipMemset(h->heavy, 0, sizeof(uint32_t) * identity.size()));
     }
     if (h->mask) {
-        const DensityOut o{h->heavy, h->nbrCount, h->mask, h->extraMask, h->wallSum, h->wallZ, h->dens, h->visc};
+        const DensityOut o{h->heavy, h->nbrCount, h->mask, h->extraMask, h->wallSum, h->wallZ, h->dens, h->visc,
+                           h->wallCells, h->wallList};
         if ((rc = dalloc(h, &h->densOut, 1))) return rc;
         SPH_HIP(h, hipMemcpy(h->densOut, &o, sizeof(o), hipMemcpyHostToDevice));
     }
@@ -930,6 +940,12 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
         SPH_HIP(h, hipMemcpy(h->spos, hSpos.data(), sizeof(float4) * Ns, hipMemcpyHostToDevice));
         SPH_HIP(h, hipMemcpy(h->stab, hStab.data(), sizeof(uint32_t) * h->Ls, hipMemcpyHostToDevice));
         SPH_HIP(h, hipMemcpy(h->wallMap, hWallMap.data(), sizeof(uint32_t) * h->wallWords, hipMemcpyHostToDevice));
+        if (!hWallCells.empty())
+            SPH_HIP(h, hipMemcpy(h->wallCells, hWallCells.data(), sizeof(uint32_t) * hWallCells.size(),
+                                 hipMemcpyHostToDevice));
+        if (!hWallList.empty())
+            SPH_HIP(h, hipMemcpy(h->wallList, hWallList.data(), sizeof(uint32_t) * hWallList.size(),
+                                 hipMemcpyHostToDevice));
         SPH_HIP(h, hipMemcpy(h->staticScene, hStaticScene.data(), sizeof(uint32_t) * Ns, hipMemcpyHostToDevice));
     }
     // Validation index lists: dynamic first/last entries per scene, then static.
@@ -1417,7 +1433,8 @@ int ensure_cw_handoffs(sph_engine* h) {
         if ((rc = dalloc(h, &h->nbrCount, std::max<uint64_t>(h->N, 1)))) return rc;
     }
     if (!h->extraMask && (rc = dalloc(h, &h->extraMask, std::max<uint64_t>(h->N, 1)))) return rc;
-    const DensityOut o{h->heavy, h->nbrCount, h->mask, h->extraMask, h->wallSum, h->wallZ, h->dens, h->visc};
+    const DensityOut o{h->heavy, h->nbrCount, h->mask, h->extraMask, h->wallSum, h->wallZ, h->dens, h->visc,
+                       h->wallCells, h->wallList};
     SPH_HIP(h, hipMemcpy(h->densOut, &o, sizeof(o), hipMemcpyHostToDevice));
     return SPH_OK;
 }

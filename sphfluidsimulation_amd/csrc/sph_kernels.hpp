@@ -42,7 +42,10 @@ struct alignas(16) SceneDev {
     // of the mesh buffer, the triangle lists at float offset binList (uint bits).
     float bx0, by0, bz0, binInv;
     uint32_t nbx, nby, nbz, binStart;
-    uint32_t binList, pad0, pad1, pad2;
+    uint32_t binList;
+    // Wall lists (sph_wall_lists.hpp): the scene's part of the cell array starts
+    // at word wlBegin, with wlMask + 1 table slots; wlMask == 0 means no table.
+    uint32_t wlBegin, wlMask, pad0;
 };
 
 struct alignas(16) BlockDesc {

@@ -9,7 +9,7 @@ divide-free remainder equals %."""
 import os
 import subprocess
 
-from sphfluidsimulation_amd.build import WALK_BIN
+from sphfluidsimulation_amd.build import WALK_BIN, WALL_LISTS_BIN
 
 
 def test_walk_visits_reference_buckets():
@@ -17,6 +17,22 @@ def test_walk_visits_reference_buckets():
     r = subprocess.run([WALK_BIN], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "0 mismatches" in r.stdout
+
+
+def test_wall_lists_keep_what_the_bucket_walk_keeps():
+    """tests/cpp/wall_lists_check.cpp, from the kernels' header
+    (sphfluidsimulation_amd/csrc/sph_wall_lists.hpp): for wall scenes on and off
+    the lattice, far from the origin (to x = 2e5, where an FP32 unit in the last
+    place is h / 6), below zero and with colliding static buckets, the walls an
+    octant's list keeps under the exact test are the walls the reference's
+    27-bucket walk keeps (SPHFunctions.cpp:455-496), in the same order with the
+    same duplicates, for particles at random and on cell and half-cell
+    boundaries. A reach short of h by 2^-7, or boxes not enlarged for FP32
+    rounding, each make it fail."""
+    assert os.path.exists(WALL_LISTS_BIN), "run __graft_entry__.build() first"
+    r = subprocess.run([WALL_LISTS_BIN], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert " 0 mismatches" in r.stdout.splitlines()[-1]
 
 
 def test_candidate_layout_positions():
