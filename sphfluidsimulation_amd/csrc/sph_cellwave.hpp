@@ -164,9 +164,11 @@ __device__ inline void settle(uint32_t (&j)[8]) {
 #define SPH_CW_BATCH_FORCE 4
 #endif
 constexpr uint32_t kBatchDensity = SPH_CW_BATCH_DENSITY;
-// Density pass, crowded buckets' extra particles.
+// Density pass, crowded buckets' extra particles, one at a time: a crowded bucket
+// mostly holds one to three extras, so larger batches are mostly padding
+// (dam-break-1m, 1000 steps: 0.3247 ms a step against 0.3290 with batches of 2).
 #ifndef SPH_CW_BATCH_CROWDED
-#define SPH_CW_BATCH_CROWDED 2
+#define SPH_CW_BATCH_CROWDED 1
 #endif
 constexpr uint32_t kBatchCrowded = SPH_CW_BATCH_CROWDED;
 constexpr uint32_t kBatchForce = SPH_CW_BATCH_FORCE;
