@@ -279,9 +279,13 @@ def main():
             e.set_profiling(None)
         return out
 
+    # In the timed region each event record costs the stream a few microseconds,
+    # so the two stages are bracketed on alternating steps (SPH_PROFILE_ROTATE):
+    # each is timed over the whole region, on every other step.
+    rotate = not args.profile_all
     if not events_after:
         for e in engines:
-            e.set_profiling(prof_stages)
+            e.set_profiling(prof_stages, rotate=rotate)
     t0 = time.perf_counter()
     run(steps)
     for e in engines:
@@ -331,7 +335,8 @@ def main():
             "launch": "one engine per scene" if plain else "batched" if len(scene_list) > 1 else "single scene",
             "dt": dt,
             "hip_graph": bool(args.graph),
-            "stage_times_from": "eager window after the timed region" if events_after else "timed region",
+            "stage_times_from": "eager window after the timed region" if events_after
+            else "timed region, density and force on alternating steps" if rotate else "timed region",
             "walk": args.walk,
             "neighbours": args.neighbours,
             "autotune": bool(args.autotune),

@@ -258,8 +258,12 @@ uint64_t sph_hash_map_size(sph_handle h, uint32_t scene);
  * (0 turns profiling off, SPH_PROFILE_ALL times every stage); sph_stage_times
  * returns the summed milliseconds and launch count per stage since the last
  * sph_set_profiling call. Steps are launched directly, not from hipGraphs,
- * while profiling is on. */
+ * while profiling is on. With SPH_PROFILE_ROTATE also set, each step brackets
+ * only one of the masked stages, taking them in turn: every event record costs
+ * the stream a few microseconds, so this halves the cost of timing two stages
+ * while each is still timed over the same run (on every other step). */
 #define SPH_PROFILE_ALL 0xFu
+#define SPH_PROFILE_ROTATE 0x10u
 int sph_set_profiling(sph_handle h, uint32_t stage_mask);
 int sph_stage_times(sph_handle h, double* ms, uint64_t* launches);
 

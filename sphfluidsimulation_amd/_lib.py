@@ -43,6 +43,7 @@ SPH_V_STATIC = 0x40
 
 STAGES = ("density", "force", "scan", "scatter")
 SPH_PROFILE_ALL = 0xF
+SPH_PROFILE_ROTATE = 0x10
 
 
 class SPHError(RuntimeError):

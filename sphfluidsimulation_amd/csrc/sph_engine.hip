@@ -130,7 +130,8 @@ struct sph_engine {
     float graphDt[4] = {0.f, 0.f, 0.f, 0.f};
 
     // Profiling.
-    uint32_t profiling = 0;  // stage mask
+    uint32_t profiling = 0;  // stage mask, SPH_PROFILE_ROTATE
+    uint32_t profStep = 0;   // with SPH_PROFILE_ROTATE: steps profiled so far
     std::vector<StageEvent> pending;
     std::vector<hipEvent_t> eventPool;
     double stageMs[SPH_STAGE_COUNT] = {0, 0, 0, 0};
@@ -403,7 +404,13 @@ Walk walk_of(uint32_t flags) {
 int enqueue_step(sph_engine* h, float dt, bool capture) {
     const int c = h->cur;
     const bool dbg = (h->flags & SPH_FLAG_RECORD_DEBUG) != 0;
-    const uint32_t prof = capture ? 0u : h->profiling;
+    uint32_t prof = capture ? 0u : h->profiling & SPH_PROFILE_ALL;
+    if (prof && (h->profiling & SPH_PROFILE_ROTATE)) {
+        // One masked stage per step, in turn.
+        uint32_t k = h->profStep++ % (uint32_t)__builtin_popcount(prof);
+        for (; k; --k) prof &= prof - 1u;
+        prof &= ~prof + 1u;
+    }
     const dim3 grid(h->numBlocks), block(kBlock);
     const dim3 gridX(((h->numBlocks + 7u) / 8u) * 8u);  // padded for the XCD-aware tile order
     // Cell-wave passes: workgroups of W waves, each wave a 64-particle slice of a block.
@@ -1400,7 +1407,8 @@ int sph_set_profiling(sph_handle h, uint32_t stage_mask) {
     SPH_HIP(h, hipStreamSynchronize(h->stream));
     int rc = collect_events(h);
     if (rc) return rc;
-    h->profiling = stage_mask & SPH_PROFILE_ALL;
+    h->profiling = stage_mask & (SPH_PROFILE_ALL | SPH_PROFILE_ROTATE);
+    h->profStep = 0;
     for (int s = 0; s < SPH_STAGE_COUNT; ++s) {
         h->stageMs[s] = 0;
         h->stageLaunches[s] = 0;

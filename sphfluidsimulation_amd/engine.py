@@ -219,8 +219,9 @@ class SimulationEngineHIP:
             self._check(rc)
         return int(bits.value)
 
-    def set_profiling(self, stages=None):
-        """stages: iterable of stage names to time with hipEvents, True for all, None/False for none."""
+    def set_profiling(self, stages=None, rotate: bool = False):
+        """stages: iterable of stage names to time with hipEvents, True for all, None/False for none.
+        rotate: bracket one of those stages per step, in turn (SPH_PROFILE_ROTATE)."""
         if stages is True:
             mask = _lib.SPH_PROFILE_ALL
         elif not stages:
@@ -229,6 +230,8 @@ class SimulationEngineHIP:
             mask = 0
             for s in stages:
                 mask |= 1 << _lib.STAGES.index(s)
+        if mask and rotate:
+            mask |= _lib.SPH_PROFILE_ROTATE
         self._check(self._lib.sph_set_profiling(self._h, mask))
 
     def stage_times(self):

@@ -39,3 +39,22 @@ def test_null_handle_is_rejected():
     assert lib.sph_synchronize(None) == _lib.SPH_E_INVALID
     assert lib.sph_read(None, None, None, None) == _lib.SPH_E_INVALID
     assert lib.sph_last_error(None) == b"invalid handle"
+
+
+def test_header_constants_match_the_host():
+    """Every #define SPH_* of include/sph_hip.h that the Python host names has the header's value."""
+    import re
+
+    text = open(_lib.HEADER_PATH).read()
+    defines = {}
+    for name, expr in re.findall(r"#define\s+(SPH_[A-Z0-9_]+)[ \t]+([^/\n]+)", text):
+        expr = re.sub(r"(?<=[0-9a-fA-F])[uU]\b", "", expr.strip())
+        if re.fullmatch(r"[0-9xXa-fA-F()<\s-]+", expr):
+            defines[name] = eval(expr, {"__builtins__": {}})  # integer literals, shifts and parentheses only
+    assert defines["SPH_PROFILE_ROTATE"] == 0x10 and defines["SPH_NOALLOC_MAX_PARTICLES"] == 1 << 20
+    checked = 0
+    for name, value in defines.items():
+        if hasattr(_lib, name):
+            assert getattr(_lib, name) == value, name
+            checked += 1
+    assert checked >= 15

@@ -817,3 +817,30 @@ def test_update_before_initialize_and_empty():
     with pytest.raises(SPHError):
         e.Update(1e-3, 1)
     e.close()
+
+
+def test_profiling_rotates_stages(s16k):
+    """SPH_PROFILE_ROTATE: each step brackets one of the masked stages, in turn, so
+    over 10 steps density and force are each timed 5 times and the others never;
+    without it every masked stage is timed every step. Timing changes no result."""
+    say("profiling rotation")
+    plain = SimulationEngineHIP(0)
+    plain.Initialize(s16k)
+    plain.Update(1e-3, 10)
+    e = SimulationEngineHIP(0)
+    e.Initialize(s16k)
+    e.set_profiling(["density", "force"], rotate=True)
+    e.Update(1e-3, 10)
+    t = e.stage_times()
+    assert {k: n for k, (ms, n) in t.items()} == {"density": 5, "force": 5, "scan": 0, "scatter": 0}
+    assert t["density"][0] > 0 and t["force"][0] > 0
+    e.set_profiling(["density", "force"])
+    e.Update(1e-3, 4)
+    t = e.stage_times()
+    assert {k: n for k, (ms, n) in t.items()} == {"density": 4, "force": 4, "scan": 0, "scatter": 0}
+    e.set_profiling(None)
+    plain.Update(1e-3, 4)
+    for a, b in zip(e.read(), plain.read()):
+        np.testing.assert_array_equal(a, b)
+    e.close()
+    plain.close()
