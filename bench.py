@@ -36,8 +36,9 @@ CONFIGS = {
     # The same scenes, one engine (and one launch per stage) per scene, over the
     # same 500-step window.
     "batch256-plain": ("batch_256", 1e-3, 500, {}),
-    # 64 scenes in mixed precision.
-    "batch64": ("batch_64_mixed", 1e-3, 200, {"graph": True, "mixed": True}),
+    # 64 scenes in mixed precision, neighbour positions stored as FP16 offsets
+    # from their cell corners.
+    "batch64": ("batch_64_mixed", 1e-3, 200, {"graph": True, "mixed": True, "stored": True}),
     # The scene the reference's application loads (triangleTestScene.json with its
     # 1,096-triangle mesh), stepped as the application steps it.
     "triangle": ("triangle_test_scene", 1e-2, 200, {}),
@@ -169,6 +170,10 @@ def main():
                          "(auto: lists up to 1.5M particles, masks above)")
     ap.add_argument("--mixed", action="store_true",
                     help="mixed precision: FP16-offset candidate filter, Kahan-compensated neighbour sums")
+    ap.add_argument("--stored", action="store_true",
+                    help="mixed precision with the FP16 offsets stored per particle (implies --mixed)")
+    ap.add_argument("--on-the-fly", action="store_true",
+                    help="mixed precision with the FP16 offsets formed from the positions (overrides a config's --stored)")
     ap.add_argument("--autotune", action="store_true",
                     help="after initialize, let the engine time every walk (and the default walk's launch "
                          "shape) and keep the fastest")
@@ -191,7 +196,8 @@ def main():
 
     from sphfluidsimulation_amd import SimulationEngineHIP, scenes
     from sphfluidsimulation_amd._lib import (SPH_FLAG_CELL_WALK, SPH_FLAG_CW_LISTS, SPH_FLAG_CW_MASKS,
-                                             SPH_FLAG_FP16_OFFSETS, SPH_FLAG_HIP_GRAPH, SPH_FLAG_RUN_WALK,
+                                             SPH_FLAG_FP16_OFFSETS, SPH_FLAG_FP16_STORED, SPH_FLAG_HIP_GRAPH,
+                                             SPH_FLAG_RUN_WALK,
                                              SPH_FLAG_TILE_WALK)
 
     if args.copy_probe:
@@ -203,7 +209,8 @@ def main():
 
     factory, dt, default_steps, stated = CONFIGS[args.config]
     args.graph = args.graph or stated.get("graph", False)
-    args.mixed = args.mixed or stated.get("mixed", False)
+    args.stored = (args.stored or stated.get("stored", False)) and not args.on_the_fly
+    args.mixed = args.mixed or args.stored or stated.get("mixed", False)
     steps = args.steps if args.steps is not None else default_steps
     t0 = time.perf_counter()
     if factory == "triangle_test_scene":
@@ -218,7 +225,7 @@ def main():
 
     flags = ((SPH_FLAG_HIP_GRAPH if args.graph else 0) | (SPH_FLAG_CELL_WALK if args.walk == "cells" else 0)
              | (SPH_FLAG_RUN_WALK if args.walk == "runs" else 0) | (SPH_FLAG_TILE_WALK if args.walk == "tile" else 0)
-             | (SPH_FLAG_FP16_OFFSETS if args.mixed else 0)
+             | (SPH_FLAG_FP16_OFFSETS if args.mixed else 0) | (SPH_FLAG_FP16_STORED if args.stored else 0)
              | (SPH_FLAG_CW_LISTS if args.neighbours == "lists" else 0)
              | (SPH_FLAG_CW_MASKS if args.neighbours == "masks" else 0))
     plain = args.config.endswith("-plain")
@@ -341,7 +348,8 @@ def main():
             "neighbours": args.neighbours,
             "autotune": bool(args.autotune),
             "launch_shape": "/".join(f"density {d}w, force {f}w" for d, f in shapes),
-            "precision": "mixed (fp16 filter, kahan sums)" if args.mixed else "fp32",
+            "precision": ("mixed (fp16 offsets stored per particle, kahan sums)" if args.stored
+                          else "mixed (fp16 filter, kahan sums)" if args.mixed else "fp32"),
         },
         "roofline": {
             "bound": "hbm",

@@ -45,6 +45,10 @@ extern "C" {
                                        filter tests FP16 cell offsets (a proven superset of the h-ball),
                                        every neighbour sum is Kahan-compensated FP32; kernel values
                                        stay FP32, conserved reductions FP64 */
+#define SPH_FLAG_FP16_STORED 0x200u /* mixed precision as SPH_FLAG_FP16_OFFSETS (which it implies), the
+                                       filter reading per-particle FP16 offsets from the particle's own
+                                       cell corner, 8 bytes written by each step's scatter, instead of
+                                       forming them from the 16-byte positions */
 #define SPH_FLAG_CELL_WALK 0x8u     /* neighbour passes read every candidate from HBM, one cell at a
                                        time: the reference's loop shape, kept for comparison */
 #define SPH_FLAG_RUN_WALK 0x10u     /* neighbour passes read candidates from HBM in merged Morton runs */

@@ -31,6 +31,7 @@ SPH_FLAG_CW_LISTS = 0x40
 SPH_FLAG_CW_MASKS = 0x80
 SPH_CW_LIST_MAX = 1572864
 SPH_FLAG_PARTICLE_MAP = 0x100
+SPH_FLAG_FP16_STORED = 0x200
 SPH_NOALLOC_MAX_PARTICLES = 1 << 20
 
 SPH_V_NONFINITE = 0x1

@@ -130,6 +130,68 @@ __device__ inline void writelane4(Mask8& m, const uint64_t (&b)[4], uint32_t t) 
     }
 }
 
+// Mixed precision: for each position of the candidate layout (round pair p, lane
+// l: rounds 2p and 2p + 1), the cell offset of its bucket from the group's cell
+// as FP16 (x: dx of both rounds in the two halves, y: dy, z: dz) and, in w, each
+// round's expected cell_tag difference (low half round 2p); positions past the
+// 27 buckets take 0xFFFF, which no difference matches. One table per workgroup,
+// filled at the start of the density pass.
+template <bool kOn>
+__device__ inline uint4* slot_cells() {
+    if constexpr (kOn) {
+        __shared__ uint4 table[2 * 64];
+        return table;
+    } else {
+        return nullptr;
+    }
+}
+__device__ inline void fill_slot_cells(uint4* table) {
+    for (uint32_t t = threadIdx.x; t < 128u; t += blockDim.x) {
+        const uint32_t p = t >> 6, ln = t & 63u;
+        _Float16 d[3][2];
+        uint32_t e[2];
+        for (uint32_t u = 0; u < 2; ++u) {
+            const uint32_t b = (64u * (2u * p + u) + ln) / kSlots;
+            const int dx = (int)(b / 9u) - 1, dy = (int)((b / 3u) % 3u) - 1, dz = (int)(b % 3u) - 1;
+            const bool in = b < 27u;
+            d[0][u] = (_Float16)(in ? dx : 0);
+            d[1][u] = (_Float16)(in ? dy : 0);
+            d[2][u] = (_Float16)(in ? dz : 0);
+            e[u] = in ? cell_tag((uint32_t)dx, (uint32_t)dy, (uint32_t)dz) : 0xFFFFu;
+        }
+        table[t] = make_uint4(h2_bits(half2v{d[0][0], d[0][1]}), h2_bits(half2v{d[1][0], d[1][1]}),
+                              h2_bits(half2v{d[2][0], d[2][1]}), e[0] | (e[1] << 16));
+    }
+    __syncthreads();
+}
+
+// Stored filter: what a candidate's FP16 offsets become (one half of each
+// register, for one round). Its cell's tag difference d from the group's equals
+// its bucket's (expected): its stored offset plus the bucket's cell offset.
+// Else the candidate is a particle of another cell sharing the bucket. Within two
+// cells on every axis (each 4-bit field of d + 2 stays below 5 only for a
+// difference in -2..2 modulo 16) it becomes NaN, which passes; the walk's exact
+// test decides. Farther, its x offset gets 64 instead of the bucket's, which
+// fails the test, unless the stored offset is NaN (a NaN coordinate): that
+// candidate passes, as the reference keeps NaN distances.
+struct HalfMark {
+    uint32_t keep, set, far;  // 16-bit masks / bits
+};
+__device__ inline HalfMark mismatch_mark(uint32_t d, uint32_t expected) {
+    // d has clear guard bits: a field's carry stops in its guard bit.
+    const uint32_t plus2 = (d + 0x0842u) & kTagFields;  // 2 in every field
+    // (field | 16) - 5 keeps its guard bit only for a field >= 5 (5 in every field: 0x14A5).
+    const bool near = (((plus2 | kTagGuards) - 0x14A5u) & kTagGuards) == 0u;
+    if (d == expected) return {0xFFFFu, 0u, 0u};
+    return near ? HalfMark{0u, 0x7E00u, 0u} : HalfMark{0xFFFFu, 0u, 0xFFFFu};
+}
+constexpr uint32_t kFarOffsetX = 0x54005400u;  // 64.0 in both FP16 halves
+
+// Record j of an array of 8-byte records through a 32-bit byte offset.
+__device__ inline uint2 ld8(const uint2* __restrict__ base, uint32_t j) {
+    return *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(base) + (j << 3));
+}
+
 // Record j of an array of 16-byte records through a 32-bit byte offset (the
 // engine holds fewer than 2^28 particles of each kind), so the load takes a
 // scalar base and one 32-bit vector offset instead of 64-bit address arithmetic.
@@ -265,11 +327,6 @@ struct Acc {
     }
 };
 
-typedef _Float16 half2v __attribute__((ext_vector_type(2)));
-
-__device__ inline half2v pack_h2(float lo, float hi) { return half2v{(_Float16)lo, (_Float16)hi}; }
-__device__ inline uint32_t h2_bits(half2v v) { return __builtin_bit_cast(uint32_t, v); }
-__device__ inline half2v h2_from(uint32_t b) { return __builtin_bit_cast(half2v, b); }
 
 // Mixed-precision candidate filter. Offsets are u = (x - c) / h from the corner c
 // of the group's cell, rounded to FP16 (corner and target share the formula, so
@@ -282,7 +339,21 @@ __device__ inline half2v h2_from(uint32_t b) { return __builtin_bit_cast(half2v,
 // 1.0085. The test against kFilterR2 = 1.02 (1.0195 in FP16) keeps every
 // neighbour. Candidates far from the cell get large or infinite offsets and fail
 // it, as they should; the exact FP32 test follows in the mask walk.
+//
+// Stored offsets (SPH_FLAG_FP16_STORED). A particle's offset record
+// (offset_record) holds u = (x - c) / h from the corner c = cell * h of its own
+// cell, rounded to FP16: |u| <= 1 (a hair above), within 2^-11. A target's offset
+// from the group's corner is its own (same cell). A candidate's is its stored u
+// plus its bucket's cell offset d in {-1, 0, 1} (slot_cells), added in FP16:
+// |u + d| < 2, another 2^-11. The corners are FP32 products, so c_own - c_group
+// differs from d h by their two roundings: below 2^-23 4097 h (4.9e-4 h) for
+// cells within kStoredFilterReach of the origin, to which the stored filter is
+// kept (a farther group passes every candidate). With the FP16 difference (2^-11) the
+// error is 2^-9 + 4.9e-4 per axis, and the computed |u_c - u_t|^2 of a pair with
+// r <= h is at most 1 + 2 sqrt(3) 2.44e-3 + 3 2.44e-3^2 plus five FP16 roundings:
+// below 1.0102, so the same kFilterR2 keeps every neighbour.
 constexpr float kFilterR2 = 1.02f;
+constexpr uint32_t kStoredFilterReach = 4096u;
 
 // Candidate of round r on this lane: layout position 64 r + lane.
 struct RoundCand {
@@ -537,9 +608,10 @@ __global__ __launch_bounds__(kBlock) void k_scatter_order(const BlockDesc* __res
                                                           const uint32_t* __restrict__ idIn,
                                                           uint32_t* __restrict__ cursor, float4* __restrict__ pos,
                                                           float4* __restrict__ vel, uint32_t* __restrict__ idOut,
-                                                          uint32_t* __restrict__ heavy, uint32_t* __restrict__ order) {
+                                                          uint32_t* __restrict__ heavy, uint32_t* __restrict__ order,
+                                                          uint2* __restrict__ off16) {
     if (blockIdx.x < numBlocks)
-        scatter_block(blockIdx.x, blocks, scenes, posT, velT, idIn, cursor, pos, vel, idOut);
+        scatter_block(blockIdx.x, blocks, scenes, posT, velT, idIn, cursor, pos, vel, idOut, nullptr, off16);
     else
         tile_order_chunk(blockIdx.x - numBlocks, heavy, numBlocks, order);
 }
@@ -585,7 +657,8 @@ struct ViscSums {
 
 // kList: record neighbour lists for k_force_cw; else the neighbour masks and
 // the crowded buckets' record for k_force_cw_mask.
-template <bool kStatic, bool kDebug, bool kMixed, bool kList>
+// kStored (with kMixed): the FP16 filter reads the stored offset records.
+template <bool kStatic, bool kDebug, bool kMixed, bool kList, bool kStored>
 #ifndef SPH_CW_DENSITY_WAVES
 #define SPH_CW_DENSITY_WAVES 7
 #endif
@@ -594,7 +667,17 @@ template <bool kStatic, bool kDebug, bool kMixed, bool kList>
 #ifndef SPH_CW_DENSITY_LIST_WAVES
 #define SPH_CW_DENSITY_LIST_WAVES 6
 #endif
-__global__ __launch_bounds__(kMaxWG, kMixed ? 6 : (kList && kViscLists ? SPH_CW_DENSITY_LIST_WAVES : SPH_CW_DENSITY_WAVES)) void k_density_cw(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
+// Mixed precision (Kahan sums): 6 waves per SIMD. With stored offsets the list
+// mode takes 5 (at 6 it spills 17 registers: 1M density 232 against 220 us) and
+// the mask mode keeps 6 (5 spills outside the loops; the 64-scene batch's density
+// 3.25 against 3.38 ms at 5).
+#ifndef SPH_CW_DENSITY_MIXED_WAVES
+#define SPH_CW_DENSITY_MIXED_WAVES 6
+#endif
+#ifndef SPH_CW_DENSITY_STORED_LIST_WAVES
+#define SPH_CW_DENSITY_STORED_LIST_WAVES 5
+#endif
+__global__ __launch_bounds__(kMaxWG, kMixed ? (kStored && kList ? SPH_CW_DENSITY_STORED_LIST_WAVES : SPH_CW_DENSITY_MIXED_WAVES) : (kList && kViscLists ? SPH_CW_DENSITY_LIST_WAVES : SPH_CW_DENSITY_WAVES)) void k_density_cw(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
                                                        const SceneDev* __restrict__ scenes, float4* pos,
                                                        const float4* __restrict__ vel,
                                                        const uint32_t* __restrict__ tab,
@@ -604,7 +687,11 @@ __global__ __launch_bounds__(kMaxWG, kMixed ? 6 : (kList && kViscLists ? SPH_CW_
                                                        const uint32_t* __restrict__ order,
                                                        uint32_t* __restrict__ nbr,
                                                        const uint32_t* __restrict__ nanCnt,
-                                                       const DensityOut* __restrict__ out, uint32_t numWG) {
+                                                       const DensityOut* __restrict__ out,
+                                                       const uint2* __restrict__ off16, uint32_t numWG) {
+    static_assert(kMixed || !kStored, "stored offsets are a mixed-precision filter");
+    uint4* const slotCell = slot_cells<kStored>();
+    if constexpr (kStored) fill_slot_cells(slotCell);  // every wave, before any leaves
     WaveSlot ws;
     if (!wave_slot(numBlocks, numWG, ws)) return;  // only wave-level sync below
     WaveTables& gt = wave_tables(ws.wave);
@@ -739,32 +826,75 @@ __global__ __launch_bounds__(kMaxWG, kMixed ? 6 : (kList && kViscLists ? SPH_CW_
         [&](uint32_t, uint32_t g0, uint32_t g1, uint32_t start27, uint32_t cnt27, auto&& between) {
             // The particle itself passes its own test (r = 0); dn skips it. Empty
             // slots are masked out with the (wave-uniform) slot masks.
+            // The candidates' positions; with stored offsets their offset records
+            // (8 bytes instead of 16), and positions only for the FP32 test.
             float4 xc[4];
+            uint2 rc[4];
             uint64_t occupied[4];
 #pragma unroll
             for (uint32_t r = 0; r < 4; ++r) {
                 const RoundCand c = round_candidate(r, lane, start27, cnt27);
-                xc[r] = ld16(pos, c.valid ? c.j : lo);
+                if constexpr (kStored)
+                    rc[r] = ld8(off16, c.valid ? c.j : lo);
+                else
+                    xc[r] = ld16(pos, c.valid ? c.j : lo);
                 occupied[r] = ballot64(c.valid);
             }
             between();
             if (kMixed) {
                 // FP16 filter, unless the group's particles span aliased cells (same
-                // Morton code, cells 1024 apart), which the FP32 test below handles.
+                // Morton code, cells 1024 apart) or, with stored offsets, lie too far
+                // from the origin for their error bound (kFilterR2).
                 const uint32_t gx = (uint32_t)__builtin_amdgcn_readlane((int)cx, (int)g0);
                 const uint32_t gy = (uint32_t)__builtin_amdgcn_readlane((int)cy, (int)g0);
                 const uint32_t gz = (uint32_t)__builtin_amdgcn_readlane((int)cz, (int)g0);
                 const bool inGroup = lane >= g0 && lane < g1;
-                if (ballot64(inGroup && (cx != gx || cy != gy || cz != gz)) == 0ull) {
-                    // The same corner formula as the lanes' own offsets above.
-                    const float gcx = (float)(int32_t)gx * h, gcy = (float)(int32_t)gy * h,
-                                gcz = (float)(int32_t)gz * h;
+                const bool nearOrigin = !kStored || ((gx + kStoredFilterReach) | (gy + kStoredFilterReach) |
+                                                     (gz + kStoredFilterReach)) < 2u * kStoredFilterReach;
+                if (nearOrigin && ballot64(inGroup && (cx != gx || cy != gy || cz != gz)) == 0ull) {
                     half2v X[2], Y[2], Z[2];  // rounds (0, 1) and (2, 3)
+                    if constexpr (kStored) {
+                        // A candidate's offset from the group's cell corner is its stored
+                        // offset plus its bucket's cell offset (slotCell), when its tag says
+                        // it lies in that cell (modulo 16 per axis). A particle of another
+                        // cell sharing the bucket is made NaN, which passes (the walk's exact
+                        // test decides), when its tag puts it within two cells of the group
+                        // on every axis, and infinite, which fails, when farther: more than
+                        // one cell from the group's cell is farther than h from its targets
+                        // except for cells met by rounding at distance h, which the two-cell
+                        // reach still passes.
+                        const uint32_t gtag = cell_tag(gx, gy, gz);
 #pragma unroll
-                    for (uint32_t p = 0; p < 2; ++p) {
-                        X[p] = pack_h2((xc[2 * p].x - gcx) * invh, (xc[2 * p + 1].x - gcx) * invh);
-                        Y[p] = pack_h2((xc[2 * p].y - gcy) * invh, (xc[2 * p + 1].y - gcy) * invh);
-                        Z[p] = pack_h2((xc[2 * p].z - gcz) * invh, (xc[2 * p + 1].z - gcz) * invh);
+                        for (uint32_t p = 0; p < 2; ++p) {
+                            // Read per group: the same for every group, but hoisted out of
+                            // the group loop these registers would stay live through it.
+                            uint32_t slotIdx = 64u * p + lane;
+                            asm volatile("" : "+v"(slotIdx));
+                            const uint4 k = slotCell[slotIdx];
+                            const uint2 a = rc[2 * p], b = rc[2 * p + 1];
+                            const uint32_t x = __builtin_amdgcn_perm(b.x, a.x, 0x05040100u);
+                            const uint32_t y = __builtin_amdgcn_perm(b.x, a.x, 0x07060302u);
+                            const uint32_t z = __builtin_amdgcn_perm(b.y, a.y, 0x05040100u);
+                            const uint32_t d0 = ((a.y >> 16) - gtag) & kTagFields;
+                            const uint32_t d1 = ((b.y >> 16) - gtag) & kTagFields;
+                            const HalfMark m0 = mismatch_mark(d0, k.w & 0xFFFFu), m1 = mismatch_mark(d1, k.w >> 16);
+                            const uint32_t keep = m0.keep | (m1.keep << 16), set = m0.set | (m1.set << 16);
+                            const uint32_t far = m0.far | (m1.far << 16);
+                            const uint32_t dx = (kFarOffsetX & far) | (k.x & ~far);
+                            X[p] = h2_from((h2_bits(h2_from(x) + h2_from(dx)) & keep) | set);
+                            Y[p] = h2_from((h2_bits(h2_from(y) + h2_from(k.y)) & keep) | set);
+                            Z[p] = h2_from((h2_bits(h2_from(z) + h2_from(k.z)) & keep) | set);
+                        }
+                    } else {
+                        // The same corner formula as the lanes' own offsets above.
+                        const float gcx = (float)(int32_t)gx * h, gcy = (float)(int32_t)gy * h,
+                                    gcz = (float)(int32_t)gz * h;
+#pragma unroll
+                        for (uint32_t p = 0; p < 2; ++p) {
+                            X[p] = pack_h2((xc[2 * p].x - gcx) * invh, (xc[2 * p + 1].x - gcx) * invh);
+                            Y[p] = pack_h2((xc[2 * p].y - gcy) * invh, (xc[2 * p + 1].y - gcy) * invh);
+                            Z[p] = pack_h2((xc[2 * p].z - gcz) * invh, (xc[2 * p + 1].z - gcz) * invh);
+                        }
                     }
                     const _Float16 R = (_Float16)kFilterR2;
                     for (uint32_t t = g0; t < g1; ++t) {
@@ -781,6 +911,13 @@ __global__ __launch_bounds__(kMaxWG, kMixed ? 6 : (kList && kViscLists ? SPH_CW_
                         }
                         writelane4(m8, b, t);
                     }
+                    return;
+                }
+                if constexpr (kStored) {
+                    // Aliased cells or far from the origin: every candidate passes (a
+                    // superset; the walk tests each exactly), so this kernel never
+                    // holds the candidates' positions.
+                    for (uint32_t t = g0; t < g1; ++t) writelane4(m8, occupied, t);
                     return;
                 }
             }

@@ -89,6 +89,7 @@ struct sph_engine {
     uint32_t* mask = nullptr;       // candidate mask from the density pass (run, tile and cell-wave walks)
     bool cwList = false;            // cell-wave: neighbour lists (else masks) between the passes
     uint64_t* extraMask = nullptr;  // cell-wave, mask mode: crowded buckets' neighbours
+    uint2* off16 = nullptr;         // cell-wave, mixed precision: FP16 offset records (offset_record)
     uint32_t* nbr = nullptr;        // cell-wave, list mode: neighbour lists (kListCap entries per particle)
     uint32_t* nbrCount = nullptr;   // cell-wave: list lengths
     DensityOut* densOut = nullptr;  // cell-wave density pass: its output pointers (device copy)
@@ -208,6 +209,7 @@ void release(sph_engine* h) {
     dfree(h->mask);
     dfree(h->nbr);
     dfree(h->extraMask);
+    dfree(h->off16);
     dfree(h->nbrCount);
     dfree(h->densOut);
     dfree(h->nanCnt);
@@ -401,6 +403,16 @@ Walk walk_of(uint32_t flags) {
 }
 
 // Enqueues one step starting from parity h->cur (does not flip it).
+// Mixed precision: the offset records of the current state (the scatter writes
+// them every step; this covers a state set otherwise). On the engine's stream.
+int refresh_offsets(sph_engine* h) {
+    if (!h->off16 || h->N == 0) return SPH_OK;
+    hipLaunchKernelGGL(k_offsets, dim3(h->numBlocks), dim3(kBlock), 0, h->stream, h->dBlocks, h->dScenes, h->pos,
+                       h->off16);
+    SPH_HIP(h, hipGetLastError());
+    return SPH_OK;
+}
+
 int enqueue_step(sph_engine* h, float dt, bool capture) {
     const int c = h->cur;
     const bool dbg = (h->flags & SPH_FLAG_RECORD_DEBUG) != 0;
@@ -444,24 +456,29 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     // Density / pressure.
     if ((rc = begin(SPH_STAGE_DENSITY))) return rc;
     const Walk walk = walk_of(h->flags);
-    const bool mixed = (h->flags & SPH_FLAG_FP16_OFFSETS) != 0;  // cell-wave walk only
+    const bool mixed = (h->flags & (SPH_FLAG_FP16_OFFSETS | SPH_FLAG_FP16_STORED)) != 0;  // cell-wave walk only
+    const bool stored = (h->flags & SPH_FLAG_FP16_STORED) != 0;
     // Per-scene counts of NaN particles in bucket 0: read this step, accumulated for the next.
     const uint32_t* nanNow = h->nanCnt + (size_t)c * h->nScenes;
     uint32_t* nanNext = h->nanCnt + (size_t)(c ^ 1) * h->nScenes;
     if (walk != Walk::Cells) {
-#define SPH_DENS_CW(S, D, X, L)                                                                                \
-        hipLaunchKernelGGL((k_density_cw<S, D, X, L>), gridD, blockD, ldsD, h->stream, h->dBlocks,            \
+#define SPH_DENS_CW(S, D, X, L, T)                                                                             \
+        hipLaunchKernelGGL((k_density_cw<S, D, X, L, T>), gridD, blockD, ldsD, h->stream, h->dBlocks,         \
                            h->numBlocks, h->dScenes, h->pos, h->vel, h->tab[c], h->spos, h->stab, h->wallMap,  \
-                           h->tileOrder, h->nbr, nanNow, h->densOut, densWG)
+                           h->tileOrder, h->nbr, nanNow, h->densOut, h->off16, densWG)
 #define SPH_DENS(S, D)                                                                                         \
-    if (walk == Walk::CellWave && mixed && h->cwList)                                                          \
-        SPH_DENS_CW(S, D, true, true);                                                                         \
+    if (walk == Walk::CellWave && stored && h->cwList)                                                         \
+        SPH_DENS_CW(S, D, true, true, true);                                                                   \
+    else if (walk == Walk::CellWave && stored)                                                                 \
+        SPH_DENS_CW(S, D, true, false, true);                                                                  \
+    else if (walk == Walk::CellWave && mixed && h->cwList)                                                     \
+        SPH_DENS_CW(S, D, true, true, false);                                                                  \
     else if (walk == Walk::CellWave && mixed)                                                                  \
-        SPH_DENS_CW(S, D, true, false);                                                                        \
+        SPH_DENS_CW(S, D, true, false, false);                                                                 \
     else if (walk == Walk::CellWave && h->cwList)                                                              \
-        SPH_DENS_CW(S, D, false, true);                                                                        \
+        SPH_DENS_CW(S, D, false, true, false);                                                                 \
     else if (walk == Walk::CellWave)                                                                           \
-        SPH_DENS_CW(S, D, false, false);                                                                       \
+        SPH_DENS_CW(S, D, false, false, false);                                                                \
     else if (walk == Walk::Tiles)                                                                              \
         hipLaunchKernelGGL((k_density_tile<S, D>), gridX, block, 0, h->stream, h->dBlocks, h->numBlocks,       \
                            h->dScenes, h->pos, h->tab[c], h->spos, h->stab, h->mask, h->dens);                 \
@@ -579,14 +596,14 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     if (walk == Walk::CellWave && h->hasStatic) {
         hipLaunchKernelGGL(k_scatter_order, dim3(h->numBlocks + 8u), block, 0, h->stream, h->dBlocks, h->numBlocks,
                            h->dScenes, h->posT, h->velT, h->ids[c], h->tab[c ^ 1], h->pos, h->vel, h->ids[c ^ 1],
-                           h->heavy, h->tileOrder);
+                           h->heavy, h->tileOrder, h->off16);
     } else if (h->pmap && !h->reorderNow) {
         // Particle-map mode between reorders: memory order kept, map refilled.
         hipLaunchKernelGGL(k_fill_map, grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->posT, h->velT,
                            h->ids[c], h->tab[c ^ 1], h->pos, h->vel, h->ids[c ^ 1], h->pmap);
     } else {
         hipLaunchKernelGGL(k_scatter, grid, block, 0, h->stream, h->dBlocks, h->dScenes, h->posT, h->velT,
-                           h->ids[c], h->tab[c ^ 1], h->pos, h->vel, h->ids[c ^ 1], h->pmap);
+                           h->ids[c], h->tab[c ^ 1], h->pos, h->vel, h->ids[c ^ 1], h->pmap, h->off16);
     }
     SPH_HIP(h, hipGetLastError());
     if ((rc = end())) return rc;
@@ -897,6 +914,8 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
             h->densWaves = h->cwList ? 4u : 2u;
             h->forceWaves = h->cwList ? 4u : 8u;
         }
+        if (walk_of(flags) == Walk::CellWave && (flags & SPH_FLAG_FP16_STORED))
+            SPH_ALLOC(h->off16, std::max<uint64_t>(N, 1));
         if (h->cwList) {
             SPH_ALLOC(h->nbr, (uint64_t)std::max<uint32_t>(h->numBlocks, 1) * kWaves * kListCap * 64u);
             SPH_ALLOC(h->nbrCount, std::max<uint64_t>(N, 1));
@@ -923,6 +942,7 @@ int sph_batch_initialize(sph_handle h, uint32_t n_scenes, const sph_params* para
         if (h->pmap) SPH_HIP(h, hipMemcpy(h->pmap, hIds.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice));
         SPH_HIP(h, hipMemcpy(h->origOfSlot, hOrig.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice));
         SPH_HIP(h, hipMemset(h->pres, 0, sizeof(float) * N));
+        if ((rc = refresh_offsets(h))) return rc;
     }
     SPH_HIP(h, hipMemcpy(h->tab[0], hTab.data(), sizeof(uint32_t) * L, hipMemcpyHostToDevice));
     hNan.resize(2ull * n_scenes, 0u);  // parity 1 starts at zero
@@ -1124,6 +1144,7 @@ int sph_write(sph_handle h, const float* pos, const float* vel) {
     hipLaunchKernelGGL(k_write, dim3((uint32_t)((N + 255) / 256)), dim3(256), 0, h->stream, h->pos, h->vel,
                        h->ids[h->cur], h->origOfSlot, (uint32_t)N, pos ? d : nullptr, vel ? d + 3 * N : nullptr);
     SPH_HIP(h, hipGetLastError());
+    if (pos && (rc = refresh_offsets(h))) return rc;
     SPH_HIP(h, hipStreamSynchronize(h->stream));
     return SPH_OK;
 }
@@ -1511,6 +1532,11 @@ int sph_autotune(sph_handle h, float dt, uint32_t trial_steps, uint32_t* chosen)
         if ((e = copy(h->ids[c], bIds, N * sizeof(uint32_t))) != hipSuccess) return e;
         if ((e = copy(h->tab[c], bTab, L * sizeof(uint32_t))) != hipSuccess) return e;
         if ((e = copy(h->nanCnt, bNan, S2 * sizeof(uint32_t))) != hipSuccess) return e;
+        if (h->off16) {  // the offset records of the restored positions
+            hipLaunchKernelGGL(k_offsets, dim3(h->numBlocks), dim3(kBlock), 0, h->stream, h->dBlocks, h->dScenes,
+                               h->pos, h->off16);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
         return hipMemsetAsync(h->tab[c ^ 1], 0, L * sizeof(uint32_t), h->stream);
     };
     hipError_t e = copy(bPos, h->pos, N * sizeof(float4));

@@ -52,6 +52,34 @@ struct alignas(16) BlockDesc {
     uint32_t scene, begin, end, pad;
 };
 
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+__device__ inline half2v pack_h2(float lo, float hi) { return half2v{(_Float16)lo, (_Float16)hi}; }
+__device__ inline uint32_t h2_bits(half2v v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ inline half2v h2_from(uint32_t b) { return __builtin_bit_cast(half2v, b); }
+
+// Mixed precision (SPH_FLAG_FP16_OFFSETS): each particle's position stored as FP16
+// offsets from the corner of its own cell, in units of h, with the low bits of
+// its cell as a tag, 8 bytes in bucket order beside pos. x: (u_x, u_y); y: u_z in
+// the low half and the tag in the high half. The tag holds the low 4 bits of
+// each cell coordinate in 5-bit fields whose top bits (kTagGuards) are set, so
+// (tag_a - tag_b) & kTagFields is the cell difference modulo 16 per axis with no
+// borrow between fields. The offsets use the corner and 1/h of the density
+// pass's own FP16 filter.
+constexpr uint32_t kTagGuards = 0x4210u;  // bits 4, 9, 14
+constexpr uint32_t kTagFields = 0x3DEFu;  // bits 0-3, 5-8, 10-13
+__device__ inline uint32_t cell_tag(uint32_t cx, uint32_t cy, uint32_t cz) {
+    return (cx & 15u) | ((cy & 15u) << 5) | ((cz & 15u) << 10);
+}
+__device__ inline uint2 offset_record(float4 p, float h) {
+    const uint32_t cx = cell_coord(p.x, h), cy = cell_coord(p.y, h), cz = cell_coord(p.z, h);
+    const float invh = 1.0f / h;
+    const float ux = (p.x - (float)(int32_t)cx * h) * invh;
+    const float uy = (p.y - (float)(int32_t)cy * h) * invh;
+    const float uz = (p.z - (float)(int32_t)cz * h) * invh;
+    const uint32_t zbits = __builtin_bit_cast(uint16_t, (_Float16)uz);
+    return make_uint2(h2_bits(pack_h2(ux, uy)), zbits | ((cell_tag(cx, cy, cz) | kTagGuards) << 16));
+}
+
 // ---------------------------------------------------------------------------
 // Wave-level helpers (wave64).
 
@@ -826,12 +854,13 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* __restrict__ da
 // step's neighbour reads are contiguous per bucket.
 // pmapOut (particle-map mode, a reordering step: SF:332-345): the state moves
 // into bucket order and the map becomes the identity.
+// off16 (mixed precision): also the particle's offset record at its new slot.
 __device__ inline void scatter_block(uint32_t blk, const BlockDesc* __restrict__ blocks,
                                      const SceneDev* __restrict__ scenes, const float4* __restrict__ posT,
                                      const float4* __restrict__ velT, const uint32_t* __restrict__ idIn,
                                      uint32_t* __restrict__ cursor, float4* __restrict__ pos,
                                      float4* __restrict__ vel, uint32_t* __restrict__ idOut,
-                                     uint32_t* __restrict__ pmapOut = nullptr) {
+                                     uint32_t* __restrict__ pmapOut, uint2* __restrict__ off16) {
     const BlockDesc bd = blocks[blk];
     const SceneDev& S = scenes[bd.scene];
     const uint32_t tBegin = S.tBegin;
@@ -854,6 +883,7 @@ __device__ inline void scatter_block(uint32_t blk, const BlockDesc* __restrict__
     vel[dst] = v;
     idOut[dst] = id;
     if (pmapOut) pmapOut[dst] = dst;
+    if (off16) off16[dst] = offset_record(p, S.h);
 }
 
 __global__ __launch_bounds__(kBlock) void k_scatter(const BlockDesc* __restrict__ blocks,
@@ -861,8 +891,20 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const BlockDesc* __restrict_
                                                     const float4* __restrict__ posT, const float4* __restrict__ velT,
                                                     const uint32_t* __restrict__ idIn, uint32_t* __restrict__ cursor,
                                                     float4* __restrict__ pos, float4* __restrict__ vel,
-                                                    uint32_t* __restrict__ idOut, uint32_t* __restrict__ pmapOut) {
-    scatter_block(blockIdx.x, blocks, scenes, posT, velT, idIn, cursor, pos, vel, idOut, pmapOut);
+                                                    uint32_t* __restrict__ idOut, uint32_t* __restrict__ pmapOut,
+                                                    uint2* __restrict__ off16) {
+    scatter_block(blockIdx.x, blocks, scenes, posT, velT, idIn, cursor, pos, vel, idOut, pmapOut, off16);
+}
+
+// The offset records of the current state (mixed precision), after the state was
+// set other than by the scatter: at initialization and after sph_write.
+__global__ __launch_bounds__(kBlock) void k_offsets(const BlockDesc* __restrict__ blocks,
+                                                    const SceneDev* __restrict__ scenes,
+                                                    const float4* __restrict__ pos, uint2* __restrict__ off16) {
+    const BlockDesc bd = blocks[blockIdx.x];
+    const uint32_t i = bd.begin + threadIdx.x;
+    if (i >= bd.end) return;
+    off16[i] = offset_record(pos[i], scenes[bd.scene].h);
 }
 
 // Particle-map mode, a step that does not reorder (SF:346-356,

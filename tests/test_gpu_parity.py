@@ -23,7 +23,7 @@ from helpers import (bucket_multisets_equal, check_buckets, expected_starts, max
                      point_triangle_distances)
 from sphfluidsimulation_amd import SPHError, SimulationEngineHIP, scenes
 from sphfluidsimulation_amd._lib import (SPH_E_STATE, SPH_FLAG_CELL_WALK, SPH_FLAG_CW_MASKS, SPH_FLAG_FP16_OFFSETS,
-                                         SPH_FLAG_HIP_GRAPH, SPH_FLAG_PARTICLE_MAP,
+                                         SPH_FLAG_FP16_STORED, SPH_FLAG_HIP_GRAPH, SPH_FLAG_PARTICLE_MAP,
                                          SPH_FLAG_RECORD_DEBUG, SPH_FLAG_RUN_WALK, SPH_FLAG_TILE_WALK,
                                          SPH_V_KEY, SPH_V_NONFINITE)
 
@@ -35,11 +35,14 @@ FP64_DENS_TOL = 1e-4
 
 # The neighbour-pass variants: cell-group waves (the default; neighbour lists
 # between the passes at these sizes, masks with SPH_FLAG_CW_MASKS), the same in
-# mixed precision, LDS-staged bucket tiles, Morton runs from HBM, and the
-# reference's cell-by-cell loop from HBM.
+# mixed precision (FP16 offsets formed from the positions, or stored per
+# particle), LDS-staged bucket tiles, Morton runs from HBM, and the reference's
+# cell-by-cell loop from HBM.
 WALKS = [pytest.param(0, id="cellwave"), pytest.param(SPH_FLAG_CW_MASKS, id="cellwave-masks"),
          pytest.param(SPH_FLAG_FP16_OFFSETS, id="mixed"),
          pytest.param(SPH_FLAG_FP16_OFFSETS | SPH_FLAG_CW_MASKS, id="mixed-masks"),
+         pytest.param(SPH_FLAG_FP16_STORED, id="mixed-stored"),
+         pytest.param(SPH_FLAG_FP16_STORED | SPH_FLAG_CW_MASKS, id="mixed-stored-masks"),
          pytest.param(SPH_FLAG_TILE_WALK, id="tile"),
          pytest.param(SPH_FLAG_RUN_WALK, id="runs"),
          pytest.param(SPH_FLAG_CELL_WALK, id="cells")]
@@ -150,7 +153,8 @@ def test_one_step_16k(s16k, walk):
     _check_step(s16k, got, ref, 1e-3)
 
 
-@pytest.mark.parametrize("mode", [pytest.param(0, id="fp32"), pytest.param(SPH_FLAG_FP16_OFFSETS, id="mixed")])
+@pytest.mark.parametrize("mode", [pytest.param(0, id="fp32"), pytest.param(SPH_FLAG_FP16_OFFSETS, id="mixed"),
+                                  pytest.param(SPH_FLAG_FP16_STORED, id="mixed-stored")])
 def test_density_vs_fp64_16k(s16k, mode):
     say(f"gpu (flags {mode}) vs fp64 oracle 16k")
     sc = s16k

@@ -30,7 +30,7 @@ import pytest
 import oracle as orc
 from helpers import check_buckets, expected_starts, max_norm_rel, max_rel
 from sphfluidsimulation_amd import SimulationEngineHIP, scenes
-from sphfluidsimulation_amd._lib import (SPH_CW_LIST_MAX, SPH_FLAG_FP16_OFFSETS, SPH_FLAG_HIP_GRAPH,
+from sphfluidsimulation_amd._lib import (SPH_CW_LIST_MAX, SPH_FLAG_FP16_STORED, SPH_FLAG_HIP_GRAPH,
                                          SPH_FLAG_RECORD_DEBUG, SPH_V_NONFINITE)
 from test_gpu_parity import ACC_TOL, DENS_TOL, FP64_DENS_TOL, _check_step, _init_parity, _step_integer_parity, say
 
@@ -167,13 +167,16 @@ def test_batch256_one_step_sample(b256):
 
 
 def test_batch64_mixed_vs_fp64_at_sample_steps():
+    """The 64-scene workload as stated: neighbour positions stored as FP16 offsets from
+    their cell corners (SPH_FLAG_FP16_STORED), Kahan sums, FP64 conserved sums; error
+    against the FP64 oracle at 10 seeded sample steps of 1000."""
     scs = scenes.batch_64_mixed()
     rng = np.random.default_rng(64)
     samples = sorted(rng.choice(1000, 10, replace=False).tolist())
     say(f"batch64 mixed: {len(scs)} scenes, {sum(sc.n for sc in scs)} particles, sample steps {samples}")
     dt = 1e-3
     e = SimulationEngineHIP(0)
-    e.Initialize(scs, flags=SPH_FLAG_RECORD_DEBUG | SPH_FLAG_FP16_OFFSETS | SPH_FLAG_HIP_GRAPH)
+    e.Initialize(scs, flags=SPH_FLAG_RECORD_DEBUG | SPH_FLAG_FP16_STORED | SPH_FLAG_HIP_GRAPH)
     mass0 = e.conserved()[:, 0].copy()
     off = _scene_slices(scs)
     worst = {"density": 0.0, "pressure": 0.0, "accel": 0.0}
