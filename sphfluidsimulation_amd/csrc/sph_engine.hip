@@ -362,8 +362,11 @@ hipEvent_t take_event(sph_engine* h) {
         h->eventPool.pop_back();
         return e;
     }
+    // Timing only: no system-scope fence when the event is recorded (that fence writes
+    // back and invalidates the caches, which costs the stream time and slows the next
+    // launch), so the events change what they time as little as possible.
     hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
     return e;
 }
 
@@ -1544,8 +1547,8 @@ int sph_autotune(sph_handle h, float dt, uint32_t trial_steps, uint32_t* chosen)
     if (e == hipSuccess) e = copy(bIds, h->ids[c], N * sizeof(uint32_t));
     if (e == hipSuccess) e = copy(bTab, h->tab[c], L * sizeof(uint32_t));
     if (e == hipSuccess) e = copy(bNan, h->nanCnt, S2 * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipEventCreate(&e0);
-    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&e0, hipEventDisableSystemFence);  // timing only
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&e1, hipEventDisableSystemFence);
     if (e != hipSuccess) {
         h->err = std::string("sph_autotune: ") + hipGetErrorString(e);
         return finish(SPH_E_HIP);
