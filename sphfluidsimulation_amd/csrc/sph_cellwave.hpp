@@ -726,8 +726,8 @@ __global__ __launch_bounds__(kMaxWG, kMixed ? (kStored && kList ? SPH_CW_DENSITY
     const float invh = 1.0f / h;
     uint32_t tux = 0u, tuy = 0u, tuz = 0u;
     if (kMixed) {
-        const float ccx = (float)(int32_t)cx * h, ccy = (float)(int32_t)cy * h, ccz = (float)(int32_t)cz * h;
-        const float ux = (xi.x - ccx) * invh, uy = (xi.y - ccy) * invh, uz = (xi.z - ccz) * invh;
+        const float ux = corner_offset(xi.x, cx, h) * invh, uy = corner_offset(xi.y, cy, h) * invh,
+                    uz = corner_offset(xi.z, cz, h) * invh;
         tux = h2_bits(pack_h2(ux, ux));
         tuy = h2_bits(pack_h2(uy, uy));
         tuz = h2_bits(pack_h2(uz, uz));
@@ -886,14 +886,15 @@ __global__ __launch_bounds__(kMaxWG, kMixed ? (kStored && kList ? SPH_CW_DENSITY
                             Z[p] = h2_from((h2_bits(h2_from(z) + h2_from(k.z)) & keep) | set);
                         }
                     } else {
-                        // The same corner formula as the lanes' own offsets above.
-                        const float gcx = (float)(int32_t)gx * h, gcy = (float)(int32_t)gy * h,
-                                    gcz = (float)(int32_t)gz * h;
+                        // The same corner as the lanes' own offsets above (corner_offset).
 #pragma unroll
                         for (uint32_t p = 0; p < 2; ++p) {
-                            X[p] = pack_h2((xc[2 * p].x - gcx) * invh, (xc[2 * p + 1].x - gcx) * invh);
-                            Y[p] = pack_h2((xc[2 * p].y - gcy) * invh, (xc[2 * p + 1].y - gcy) * invh);
-                            Z[p] = pack_h2((xc[2 * p].z - gcz) * invh, (xc[2 * p + 1].z - gcz) * invh);
+                            X[p] = pack_h2(corner_offset(xc[2 * p].x, gx, h) * invh,
+                                           corner_offset(xc[2 * p + 1].x, gx, h) * invh);
+                            Y[p] = pack_h2(corner_offset(xc[2 * p].y, gy, h) * invh,
+                                           corner_offset(xc[2 * p + 1].y, gy, h) * invh);
+                            Z[p] = pack_h2(corner_offset(xc[2 * p].z, gz, h) * invh,
+                                           corner_offset(xc[2 * p + 1].z, gz, h) * invh);
                         }
                     }
                     const _Float16 R = (_Float16)kFilterR2;

@@ -73,9 +73,9 @@ __device__ inline uint32_t cell_tag(uint32_t cx, uint32_t cy, uint32_t cz) {
 __device__ inline uint2 offset_record(float4 p, float h) {
     const uint32_t cx = cell_coord(p.x, h), cy = cell_coord(p.y, h), cz = cell_coord(p.z, h);
     const float invh = 1.0f / h;
-    const float ux = (p.x - (float)(int32_t)cx * h) * invh;
-    const float uy = (p.y - (float)(int32_t)cy * h) * invh;
-    const float uz = (p.z - (float)(int32_t)cz * h) * invh;
+    const float ux = corner_offset(p.x, cx, h) * invh;
+    const float uy = corner_offset(p.y, cy, h) * invh;
+    const float uz = corner_offset(p.z, cz, h) * invh;
     const uint32_t zbits = __builtin_bit_cast(uint16_t, (_Float16)uz);
     return make_uint2(h2_bits(pack_h2(ux, uy)), zbits | ((cell_tag(cx, cy, cz) | kTagGuards) << 16));
 }

@@ -16,6 +16,15 @@ __host__ __device__ inline uint32_t cell_coord(float x, float h) {
     return (uint32_t)(int32_t)floorf(x / h);
 }
 
+// x minus the corner cell * h of a cell, the product rounded to FP32 first and
+// never fused with the subtraction, so that every caller forms the same corner
+// (the candidate filters rely on targets and candidates sharing it).
+__host__ __device__ inline float corner_offset(float x, uint32_t cell, float h) {
+#pragma clang fp contract(off)
+    const float c = (float)(int32_t)cell * h;
+    return x - c;
+}
+
 // Hash of a cell's Morton code for the wall-cell bitmap (xor-shift-multiply
 // rounds). Cells whose codes alias (1024 cells apart on an axis) share a bit,
 // which only makes the bitmap's filter looser.
