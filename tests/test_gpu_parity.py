@@ -323,6 +323,21 @@ def test_negative_coordinates_wrap(s16k, walk):
 
 
 @pytest.mark.parametrize("walk", WALKS)
+def test_far_from_origin(s16k, walk):
+    # Cells 4,200 to 6,000 from the origin on every axis, both signs: beyond the
+    # 4,096 cells within which the stored FP16 offsets filter candidates (farther
+    # groups pass every candidate to the exact test), with the FP16 offsets formed
+    # from positions near 500 (FP32 spacing 6e-5 there).
+    say(f"far from the origin (walk flags {walk})")
+    shift = np.float32([500.0, -600.0, 420.0])
+    sc = scenes.Scene(s16k.params, (s16k.pos + shift).astype(np.float32), s16k.vel,
+                      (s16k.static_pos + shift).astype(np.float32), name="far-16k")
+    _init_parity(sc)
+    got, ref = _one_step(sc, 1e-3, threads=4, walk=walk)
+    _check_step(sc, got, ref, 1e-3)
+
+
+@pytest.mark.parametrize("walk", WALKS)
 def test_debug_smallest_scene(walk):
     say(f"debugSmallestScene (h = 1, M = 16, colliding buckets; walk flags {walk})")
     sc = scenes.debug_smallest()
