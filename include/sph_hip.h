@@ -272,9 +272,10 @@ int sph_set_profiling(sph_handle h, uint32_t stage_mask);
 int sph_stage_times(sph_handle h, double* ms, uint64_t* launches);
 
 /* Autotuner with no counterpart in the reference (whose engines have one fixed
- * neighbour loop, SPHFunctions.cpp:397-404): times every neighbour-pass variant
- * (cell-group waves with neighbour lists and with neighbour masks, Morton runs,
- * LDS bucket tiles) for 2 warm-up steps plus trial_steps (rounded up to an even
+ * neighbour loop, SPHFunctions.cpp:397-404): times the neighbour-pass variants
+ * (cell-group waves with neighbour lists and with neighbour masks, Morton runs;
+ * the LDS bucket tiles only when they are the configured walk, being many times
+ * slower elsewhere) for 2 warm-up steps plus trial_steps (rounded up to an even
  * count, at least 2) steps of dt on the current state, keeps the fastest for
  * later updates (the configured variant unless another is more than 3% faster)
  * and, for the cell-group waves, their launch shape the same way (waves per

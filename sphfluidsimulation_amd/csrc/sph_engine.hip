@@ -1587,15 +1587,19 @@ int sph_autotune(sph_handle h, float dt, uint32_t trial_steps, uint32_t* chosen)
         return SPH_OK;
     };
     // The engine's configured variant first, then the others among the cell-wave
-    // walk with lists and with masks, the Morton runs and the bucket tiles. A
-    // variant replaces the best so far only when it is more than 3% faster: the
-    // trial steps are few, and the configured choice reflects the whole run
-    // (the lists' advantage at 1M grows as the fluid compresses).
+    // walk with lists and with masks and the Morton runs. The bucket tiles, many
+    // times slower than the cell-wave walk on every workload measured, are timed
+    // only when they are the configured walk. A variant replaces the best so far
+    // only when it is more than 3% faster: the trial steps are few, and the
+    // configured choice reflects the whole run (the lists' advantage at 1M grows
+    // as the fluid compresses).
     uint32_t candidates[4] = {0u, 0u, SPH_FLAG_RUN_WALK, SPH_FLAG_TILE_WALK};
     bool candLists[4] = {true, false, false, false};
+    int numCandidates = 3;
     {
         const uint32_t cw = h->flags & walkBits;
         int cur = cw == SPH_FLAG_RUN_WALK ? 2 : cw == SPH_FLAG_TILE_WALK ? 3 : (h->cwList ? 0 : 1);
+        if (cur == 3) numCandidates = 4;
         std::swap(candidates[0], candidates[cur]);
         std::swap(candLists[0], candLists[cur]);
     }
@@ -1604,7 +1608,7 @@ int sph_autotune(sph_handle h, float dt, uint32_t trial_steps, uint32_t* chosen)
     bool bestLists = h->cwList;
     float bestMs = 0.f;
     bool first = true;
-    for (int ci = 0; ci < 4; ++ci) {
+    for (int ci = 0; ci < numCandidates; ++ci) {
         const uint32_t w = candidates[ci];
         h->flags = base | w;
         if (w == 0u) h->cwList = candLists[ci];
