@@ -51,7 +51,7 @@ at one bucket per wall (SimulationEngine.h:79-114), as the HIP engine does;
 are read and ignored, with a warning when the dynamic one is not 2.
 
 Command line:
-    python -m sphfluidsimulation_amd.profiles FILE [--out DIR] [--max-updates N] [--device D]
+    python -m sphfluidsimulation_amd.profiles FILE [--out DIR] [--max-updates N] [--device D] [--every-step]
 """
 from __future__ import annotations
 
@@ -243,9 +243,11 @@ def profile_report(p: Profile, systems: list[str], rows: list[list[float]]) -> s
     return out
 
 
-def run_profile(p: Profile, device: int = 0, max_updates: int | None = None, log=None) -> list[float]:
+def run_profile(p: Profile, device: int = 0, max_updates: int | None = None, log=None,
+                every_step: bool = False) -> list[float]:
     """Runs a profile on the HIP engine; returns the time per step (ms) of each
-    update, each update synchronised before the clock stops."""
+    update, each update synchronised before the clock stops. every_step: run the
+    engine's default mode (reordering every step) whatever the profile's interval."""
     from ._lib import SPH_FLAG_PARTICLE_MAP, SPH_V_NONFINITE
     from .engine import SimulationEngineHIP
 
@@ -254,7 +256,7 @@ def run_profile(p: Profile, device: int = 0, max_updates: int | None = None, log
         n_upd = min(n_upd, int(max_updates))
     e = SimulationEngineHIP(device)
     try:
-        if p.reorder_time_interval is not None:
+        if p.reorder_time_interval is not None and not every_step:
             e.Initialize(p.scene, flags=SPH_FLAG_PARTICLE_MAP)
             e.set_reorder_interval(p.reorder_time_interval)
         else:
@@ -276,7 +278,7 @@ def run_profile(p: Profile, device: int = 0, max_updates: int | None = None, log
 
 
 def run_profiles(path: str, out_dir: str = ".", device: int = 0, max_updates: int | None = None,
-                 log=print) -> list[str]:
+                 log=print, every_step: bool = False) -> list[str]:
     """Runs every profile of a file on the HIP engine and writes each report to
     its outputFilePath under out_dir; returns the paths written."""
     written = []
@@ -284,8 +286,8 @@ def run_profiles(path: str, out_dir: str = ".", device: int = 0, max_updates: in
         if log:
             log(f'[profile] "{p.name}": {p.scene.n} dynamic, {p.scene.n_static} static particles, '
                 f"{update_count(p)} updates of {p.steps_per_update} steps at {p.step_time} s, reorder interval "
-                f"{p.reorder_time_interval} s")
-        rows = run_profile(p, device=device, max_updates=max_updates, log=log)
+                f"{'every step' if every_step or p.reorder_time_interval is None else p.reorder_time_interval}")
+        rows = run_profile(p, device=device, max_updates=max_updates, log=log, every_step=every_step)
         dst = os.path.join(out_dir, p.output_file_path)
         os.makedirs(os.path.dirname(os.path.abspath(dst)), exist_ok=True)
         with open(dst, "w", encoding="utf-8") as fh:
@@ -300,9 +302,11 @@ def main(argv=None) -> int:
     ap.add_argument("--out", default=".", help="directory the profiles' outputFilePath are taken relative to")
     ap.add_argument("--device", type=int, default=0)
     ap.add_argument("--max-updates", type=int, default=None, help="stop each profile after this many updates")
+    ap.add_argument("--every-step", action="store_true",
+                    help="the engine's default mode (reorder every step) instead of the profile's interval")
     args = ap.parse_args(argv)
     for path in run_profiles(args.file, out_dir=args.out, device=args.device, max_updates=args.max_updates,
-                             log=lambda m: print(m, flush=True)):
+                             log=lambda m: print(m, flush=True), every_step=args.every_step):
         print(f"[profile] wrote {path}", flush=True)
     return 0
 
