@@ -203,9 +203,14 @@ def parse_profiles(doc, default_name: str = "profile") -> list[Profile]:
             w = f"profile '{name}': "
             sp = _get(p, "systemParameters", None, w)
             interval = _check_system(sp, name)
-            out.append(Profile(name, str(_get(p, "outputFilePath", None, w)),
-                               _get(p, "simulationDuration", "float", w), _get(p, "simulationStepTime", "float", w),
-                               _get(p, "stepsPerUpdate", "uint", w), sp, interval, int(sp.get("bufferCount", 3))))
+            duration = _get(p, "simulationDuration", "float", w)
+            dt = _get(p, "simulationStepTime", "float", w)
+            spu = _get(p, "stepsPerUpdate", "uint", w)
+            if not (dt > 0.0 and duration >= 0.0 and spu >= 1 and math.isfinite(duration)):
+                raise ProfileError(f"{w}needs simulationStepTime > 0, a finite simulationDuration >= 0 and "
+                                   "stepsPerUpdate >= 1")
+            out.append(Profile(name, str(_get(p, "outputFilePath", None, w)), duration, dt, spu, sp, interval,
+                               int(sp.get("bufferCount", 3))))
     return out
 
 

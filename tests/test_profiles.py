@@ -164,6 +164,16 @@ def test_invalid_profiles_are_rejected(mutate, message):
         parse_profiles("{not json")
 
 
+@pytest.mark.parametrize("key,value", [("simulationStepTime", 0.0), ("stepsPerUpdate", 0),
+                                       ("simulationDuration", -1.0)])
+def test_invalid_stepping_is_rejected(key, value):
+    with open(os.path.join(PROFILES, "systemProfilingProfiles.json")) as fh:
+        doc = json.load(fh)
+    doc["profiles"][0][key] = value
+    with pytest.raises(ProfileError, match="stepsPerUpdate"):
+        parse_profiles(doc)
+
+
 @pytest.mark.gpu
 def test_wave_profile_runs_on_the_hip_engine(tmp_path):
     print("[gpu-test] systemProfilingProfiles.json on the HIP engine, 2 updates", flush=True)
