@@ -1,5 +1,7 @@
 """Long-run behaviour of the HIP engine against the oracle: 1000 steps of the
-16,000-particle dam break at dt = 1e-3.
+16,000-particle dam break at dt = 1e-3 and, with SPH_LONG_TESTS=1 in the
+environment (several minutes of oracle time), 1000 steps of the
+1,048,576-particle dam break.
 
 Individual particles diverge chaotically between any two engines that sum in
 different orders (the reference CPU engine run with 1 and with several threads
@@ -7,8 +9,9 @@ already does), so the comparison is on conserved and aggregate quantities:
   * total mass: exactly equal at every checkpoint (FP64 sums of a per-particle
     FP32 mass);
   * kinetic energy and centre of mass: the GPU-vs-oracle difference stays
-    within ENVELOPE times the oracle's own spread (1 thread against 8 threads)
-    plus a floor (KE: 2% of the oracle's KE; centre of mass: 2 mm).
+    within ENVELOPE times the oracle's own spread (1 thread against 8 threads;
+    at 1M, 8 against 16) plus a floor (KE: 2% of the oracle's KE; centre of
+    mass: 2 mm).
 The reference's RandomDirection is NaN for a coincident pair whose noise is
 negative (SPHFunctions.cpp:140-155, C++ signed modf), and any run can meet such
 a pair by chance; the NaN then spreads to every particle that visits the NaN
@@ -16,8 +19,11 @@ particle's bucket (the reference keeps NaN distances), in the oracle and the
 GPU engine alike. The GPU run may be non-finite only at checkpoints where an
 oracle run is too; checkpoints where any of the three runs is no longer finite
 are left out of the KE and centre-of-mass comparison; at least 8 of the 11 must
-remain.
+remain (at 1M, where the first NaN comes after 400 to 700 steps, at least 5, with
+no order of onset required: see test_dam_break_1m_1000_steps).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -35,15 +41,15 @@ ENVELOPE = 10.0
 def run_oracle(sc, threads):
     o = orc.Oracle(sc.params, sc.pos, sc.vel, sc.static_pos, threads=threads)
     out = [o.conserved()]
-    for _ in range(STEPS // EVERY):
+    for k in range(STEPS // EVERY):
         o.update(1e-3, EVERY)
         out.append(o.conserved())
+        print(f"[gpu-test] oracle ({threads} threads): step {(k + 1) * EVERY}", flush=True)
     o.close()
     return np.array(out)
 
 
-def test_dam_break_16k_1000_steps():
-    sc = scenes.cpu_16k()
+def _trajectory(sc, threads_a, threads_b, strict_onset=True, min_finite=8):
     e = SimulationEngineHIP(0)
     e.Initialize(sc)
     gpu = [e.conserved()[0]]
@@ -56,24 +62,45 @@ def test_dam_break_16k_1000_steps():
         assert bool(bits & SPH_V_NONFINITE) == (not np.all(np.isfinite(gpu[-1][4:8])))
     gpu = np.array(gpu)
     e.close()
-    r8 = run_oracle(sc, 8)
-    r1 = run_oracle(sc, 1)
-    print("[gpu-test] step  KE_gpu  KE_orc8  KE_orc1  COMy_gpu  COMy_orc8", flush=True)
+    ra = run_oracle(sc, threads_a)
+    rb = run_oracle(sc, threads_b)
+    print(f"[gpu-test] step  KE_gpu  KE_orc{threads_a}  KE_orc{threads_b}  COMy_gpu  COMy_orc{threads_a}", flush=True)
     for k in range(gpu.shape[0]):
-        print(f"[gpu-test] {k * EVERY:5d} {gpu[k, 4]:.6g} {r8[k, 4]:.6g} {r1[k, 4]:.6g} "
-              f"{gpu[k, 6]:.6g} {r8[k, 6]:.6g}", flush=True)
+        print(f"[gpu-test] {k * EVERY:5d} {gpu[k, 4]:.6g} {ra[k, 4]:.6g} {rb[k, 4]:.6g} "
+              f"{gpu[k, 6]:.6g} {ra[k, 6]:.6g}", flush=True)
     # mass: exact
-    np.testing.assert_array_equal(gpu[:, 0], r8[:, 0])
-    np.testing.assert_array_equal(gpu[:, 0], r1[:, 0])
+    np.testing.assert_array_equal(gpu[:, 0], ra[:, 0])
+    np.testing.assert_array_equal(gpu[:, 0], rb[:, 0])
     # kinetic energy and centre of mass: within the oracle's own spread
-    fin8, fin1 = np.all(np.isfinite(r8[:, 4:8]), axis=1), np.all(np.isfinite(r1[:, 4:8]), axis=1)
+    fin_a, fin_b = np.all(np.isfinite(ra[:, 4:8]), axis=1), np.all(np.isfinite(rb[:, 4:8]), axis=1)
     fin_gpu = np.all(np.isfinite(gpu[:, 4:8]), axis=1)
-    # a NaN only the GPU produced is a GPU bug, not the reference's coincident-pair NaN
-    assert np.all(fin_gpu | ~(fin8 & fin1)), "the GPU run went non-finite where both oracle runs stayed finite"
-    ok = fin8 & fin1 & fin_gpu
-    assert ok.sum() >= 8, "a run went non-finite too early for a comparison"
-    g, a, b = gpu[ok], r8[ok], r1[ok]
+    if strict_onset:
+        # a NaN only the GPU produced is a GPU bug, not the reference's coincident-pair NaN
+        assert np.all(fin_gpu | ~(fin_a & fin_b)), "the GPU run went non-finite where both oracle runs stayed finite"
+    ok = fin_a & fin_b & fin_gpu
+    assert ok.sum() >= min_finite, "a run went non-finite too early for a comparison"
+    g, a, b = gpu[ok], ra[ok], rb[ok]
     spread_ke = np.abs(a[:, 4] - b[:, 4])
     spread_com = np.abs(a[:, 5:8] - b[:, 5:8])
     assert np.all(np.abs(g[:, 4] - a[:, 4]) <= ENVELOPE * spread_ke + 0.02 * np.abs(a[:, 4]) + 1e-12)
     assert np.all(np.abs(g[:, 5:8] - a[:, 5:8]) <= ENVELOPE * spread_com + 2e-3)
+
+
+def test_dam_break_16k_1000_steps():
+    _trajectory(scenes.cpu_16k(), 8, 1)
+
+
+@pytest.mark.skipif(os.environ.get("SPH_LONG_TESTS") != "1",
+                    reason="several minutes of oracle time; set SPH_LONG_TESTS=1 to run")
+def test_dam_break_1m_1000_steps():
+    # After about 0.4 s of the 1M run, spray particles under negative pressure
+    # (p = k (rho - rho0) < 0, which attracts) pull together in pairs until their
+    # FP32 positions coincide exactly, and the one of a pair with negative noise
+    # turns NaN, as the reference's does. One GPU run: particles 238827 and 443836
+    # fell side by side 2e-5 apart from step 575 on and coincided at step 614. When
+    # the first pair coincides is chance, since every run's summation order
+    # differs (the oracle's too, its buckets being filled by racing threads): over
+    # six oracle runs the first NaN came after steps 500 to 700, over five GPU runs
+    # after steps 400 to 615. So the steps 0 to 400 are compared, and no order of
+    # onset is required.
+    _trajectory(scenes.gpu_1m(), 16, 8, strict_onset=False, min_finite=5)
