@@ -67,16 +67,16 @@ __device__ inline WaveTables& wave_tables(uint32_t wave) {
 // results do not depend on the shape. The wave index goes through readfirstlane:
 // the compiler then knows it, the block and the scene are wave-uniform and keeps
 // them (and every load through them) in scalar registers. Returns false for waves
-// past the last block. interleave: the workgroups take tiles by xcd_tile_runs.
+// past the last block. run > 0: the workgroups take tiles by xcd_tile_runs.
 struct WaveSlot {
     uint32_t tile;   // block index in launch order
     uint32_t slice;  // the block's 64-particle slice this wave takes
     uint32_t wave;   // the wave's index in its workgroup
 };
-__device__ inline bool wave_slot(uint32_t numBlocks, uint32_t numWG, WaveSlot& ws, bool interleave = false) {
+__device__ inline bool wave_slot(uint32_t numBlocks, uint32_t numWG, WaveSlot& ws, uint32_t run = 0u) {
     const uint32_t W = blockDim.x >> 6;
     ws.wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t gw = (interleave ? xcd_tile_runs(blockIdx.x) : xcd_tile(blockIdx.x, numWG)) * W + ws.wave;
+    const uint32_t gw = (run ? xcd_tile_runs(blockIdx.x, run) : xcd_tile(blockIdx.x, numWG)) * W + ws.wave;
     ws.tile = gw / kWaves;
     ws.slice = gw % kWaves;
     return ws.tile < numBlocks;
@@ -1185,9 +1185,9 @@ __global__ __launch_bounds__(kMaxWG) void k_force_cw(const BlockDesc* __restrict
                                                      const uint32_t* __restrict__ ids, const float* __restrict__ tris,
                                                      float dt, float4* __restrict__ posT, float4* __restrict__ velT,
                                                      uint32_t* __restrict__ counts, float4* __restrict__ accOut,
-                                                     uint32_t numWG, bool interleave) {
+                                                     uint32_t numWG, uint32_t run) {
     WaveSlot ws;
-    if (!wave_slot(numBlocks, numWG, ws, interleave)) return;
+    if (!wave_slot(numBlocks, numWG, ws, run)) return;
     const uint32_t blk = ws.tile;
     const BlockDesc bd = blocks[blk];
     const SceneDev& S = scenes[bd.scene];
@@ -1243,9 +1243,9 @@ __global__ __launch_bounds__(kMaxWG, kMixed ? 6 : 8) void k_force_cw_mask(
     const uint64_t* __restrict__ extraMask, const uint32_t* __restrict__ nanCnt, uint32_t* __restrict__ nanNext,
     const uint32_t* __restrict__ ids, const float* __restrict__ tris, float dt, float4* __restrict__ posT,
     float4* __restrict__ velT, uint32_t* __restrict__ counts, float4* __restrict__ accOut, uint32_t numWG,
-    bool interleave) {
+    uint32_t run) {
     WaveSlot ws;
-    if (!wave_slot(numBlocks, numWG, ws, interleave)) return;
+    if (!wave_slot(numBlocks, numWG, ws, run)) return;
     WaveTables& gt = wave_tables(ws.wave);
     const uint32_t blk = ws.tile;
     const BlockDesc bd = blocks[blk];

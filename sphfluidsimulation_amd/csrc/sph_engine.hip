@@ -432,12 +432,14 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
     auto wgs = [&](uint32_t W) { return (h->numBlocks * kWaves + W - 1u) / W; };
     const uint32_t densWG = wgs(h->densWaves), forceWG = wgs(h->forceWaves);
     const dim3 gridD(((densWG + 7u) / 8u) * 8u), blockD(64u * h->densWaves);
-    // The force pass of a single scene takes its tiles with the XCDs interleaved
-    // (xcd_tile_runs): its work per tile varies along the particle range, and
-    // contiguous eighths left the XCDs unevenly loaded. A batch keeps each XCD on
-    // contiguous scenes, whose data then stays in that XCD's L2.
-    const bool forceInterleave = h->nScenes == 1;
-    const uint32_t padF = forceInterleave ? 8u * kXcdRun : 8u;
+    // The force pass of a single scene takes its tiles with the XCDs interleaved,
+    // kXcdRunsPerXcd runs on each (xcd_tile_runs): its work per tile varies along
+    // the particle range, and contiguous eighths left the XCDs unevenly loaded. A
+    // batch keeps each XCD on contiguous scenes, whose data then stays in that
+    // XCD's L2.
+    const uint32_t forceRun =
+        h->nScenes == 1 ? std::max(1u, (forceWG + 8u * kXcdRunsPerXcd - 1u) / (8u * kXcdRunsPerXcd)) : 0u;
+    const uint32_t padF = forceRun ? 8u * forceRun : 8u;
     const dim3 gridF(((forceWG + padF - 1u) / padF) * padF), blockF(64u * h->forceWaves);
     const size_t ldsD = sizeof(WaveTables) * h->densWaves, ldsF = sizeof(WaveTables) * h->forceWaves;
     StageEvent ev{};
@@ -538,12 +540,12 @@ int enqueue_step(sph_engine* h, float dt, bool capture) {
         hipLaunchKernelGGL((k_force_cw<S, D, T, X>), gridF, blockF, 0, h->stream, h->dBlocks, h->numBlocks,      \
                            h->dScenes, h->pos, h->vel, h->tab[c], h->wallSum, h->wallZ, h->visc, h->nbr,          \
                            h->nbrCount, nanNext, h->ids[c], h->tris, dt, h->posT, h->velT, h->tab[c ^ 1], h->acc, \
-                           forceWG, forceInterleave)
+                           forceWG, forceRun)
 #define SPH_FORCE_MASK(S, D, T, X)                                                                                \
         hipLaunchKernelGGL((k_force_cw_mask<S, D, T, X>), gridF, blockF, ldsF, h->stream, h->dBlocks,            \
                            h->numBlocks, h->dScenes, h->pos, h->vel, h->tab[c], h->wallSum, h->wallZ, h->visc,    \
                            h->mask, h->extraMask, nanNow, nanNext, h->ids[c], h->tris, dt, h->posT, h->velT,      \
-                           h->tab[c ^ 1], h->acc, forceWG, forceInterleave)
+                           h->tab[c ^ 1], h->acc, forceWG, forceRun)
 #define SPH_FORCE(S, D, T)                                                                                        \
     do {                                                                                                          \
     if (walk == Walk::CellWave && mixed && h->cwList)                                                             \

@@ -436,16 +436,18 @@ __device__ inline uint32_t xcd_tile(uint32_t b, uint32_t numTiles) {
     return (b & 7u) * chunk + (b >> 3);
 }
 
-// The same with the XCDs interleaved in runs of kXcdRun tiles: XCD x takes runs
-// x, x + 8, x + 16, ... (the launch padded to a multiple of 8 kXcdRun), so each
-// XCD's share spreads over the whole particle range, which evens out the XCDs'
-// work when it varies along the range, while each run of consecutive tiles still
+// The same with the XCDs interleaved in runs of `run` tiles: XCD x takes runs x,
+// x + 8, x + 16, ... (the launch padded to a multiple of 8 run), so each XCD's
+// share spreads over the whole particle range, which evens out the XCDs' work
+// when it varies along the range, while each run of consecutive tiles still
 // shares one XCD's L2.
-constexpr uint32_t kXcdRun = 64;
-__device__ inline uint32_t xcd_tile_runs(uint32_t b) {
+__device__ inline uint32_t xcd_tile_runs(uint32_t b, uint32_t run) {
     const uint32_t x = b & 7u, i = b >> 3;
-    return ((i / kXcdRun) * 8u + x) * kXcdRun + (i % kXcdRun);
+    return ((i / run) * 8u + x) * run + (i % run);
 }
+// Runs per XCD of an interleaved launch: eight (measured best against fixed runs
+// of 32 to 256 workgroups at 1M and 8M).
+constexpr uint32_t kXcdRunsPerXcd = 8;
 
 template <bool kStatic, bool kDebug>
 __global__ __launch_bounds__(kBlock) void k_density_runs(const BlockDesc* __restrict__ blocks, uint32_t numBlocks,
