@@ -78,19 +78,23 @@ KERNELS = {
 }
 
 
-def load_traffic(kernel: str, workload: str):
+def load_traffic(kernel: str, workload: str, steps: int):
     """HBM bytes per launch of a kernel from a committed rocprofv3 --pmc summary
-    (profiles/<kernel>_<workload>_pmc.md or _trace.md, line 'traffic_bytes: X'), if any."""
-    for kind in ("pmc", "trace"):
-        path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_{kind}.md")
-        if not os.path.exists(path):
-            continue
-        for line in open(path):
-            if line.startswith("traffic_bytes:"):
-                try:
-                    return float(line.split(":", 1)[1])
-                except ValueError:
-                    return None
+    (line 'traffic_bytes: X'), if any: the summary of this step window
+    (profiles/<kernel>_<workload>-<steps>-steps_pmc.md), else the workload's
+    (profiles/<kernel>_<workload>_pmc.md or _trace.md). The density pass's bytes
+    grow along a run, so the window matters."""
+    for name in (f"{workload}-{steps}-steps", workload):
+        for kind in ("pmc", "trace"):
+            path = os.path.join(ROOT, "profiles", f"{kernel}_{name}_{kind}.md")
+            if not os.path.exists(path):
+                continue
+            for line in open(path):
+                if line.startswith("traffic_bytes:"):
+                    try:
+                        return float(line.split(":", 1)[1])
+                    except ValueError:
+                        return None
     return None
 
 
@@ -319,7 +323,7 @@ def main():
     workload = scene_list[0].name if len(scene_list) == 1 else f"{args.config}-{len(scene_list)}-scenes"
     walk_kernels = KERNELS.get(args.walk, {})
     kernel = walk_kernels.get(dom, "k_" + dom)
-    traffic = load_traffic(kernel, workload)
+    traffic = load_traffic(kernel, workload, steps)
     b_step = step_bytes(n_total, M_total, Ms_total)
     result = {
         "metric": METRIC,
