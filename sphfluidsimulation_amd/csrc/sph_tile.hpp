@@ -7,7 +7,8 @@
 // then walks its 27 neighbour cells in the reference's order (SF:397-404),
 // looks each cell's bucket up in the map and reads that bucket's candidates
 // from LDS. A bucket that two of the 27 cells collide on is walked twice, as
-// the reference does (P5 of the survey); a bucket that does not fit in LDS, or
+// the reference does (SF:397-404 visits each cell's bucket; with M below the
+// Morton range two cells can share one); a bucket that does not fit in LDS, or
 // did not fit in the map, is read from HBM instead, so every M works.
 //
 // Particles in the same cell share their 27 buckets, so only the first

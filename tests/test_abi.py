@@ -27,7 +27,7 @@ def test_no_device_is_an_error_not_a_fallback():
     lib = _lib.load()
     h = ctypes.c_void_p()
     rc = lib.sph_create(ctypes.byref(h), 0)
-    if rc == _lib.SPH_OK:  # running on a GPU box
+    if rc == _lib.SPH_OK:  # a HIP device is present
         lib.sph_destroy(h)
         pytest.skip("a HIP device is present")
     assert rc == _lib.SPH_E_NO_DEVICE

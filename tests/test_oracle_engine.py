@@ -6,7 +6,7 @@ Checks used (none retypes the oracle's loops):
     order of a one-thread run;
   * density: all-pairs numpy sum where each pair counts once per visit of the
     neighbour's bucket among the 27 bucket visits of the particle, so colliding
-    buckets (P5 of the survey: M not covering the Morton range) are double
+    buckets (M not covering the Morton range, so two cells share one) are double
     counted exactly as the reference's walk does;
   * forces: closed-form accelerations of a particle pair and of a particle beside
     one wall, an all-pairs numpy acceleration weighted by the same bucket-visit
