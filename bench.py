@@ -78,24 +78,37 @@ KERNELS = {
 }
 
 
-def load_traffic(kernel: str, workload: str, steps: int):
-    """HBM bytes per launch of a kernel from a committed rocprofv3 --pmc summary
-    (line 'traffic_bytes: X'), if any: the summary of this step window
-    (profiles/<kernel>_<workload>-<steps>-steps_pmc.md), else the workload's
-    (profiles/<kernel>_<workload>_pmc.md or _trace.md). The density pass's bytes
-    grow along a run, so the window matters."""
+def _profile_value(kernel: str, workload: str, steps: int, kinds, key: str):
+    """Value of the line '<key>: X' of a committed rocprofv3 summary, and its file:
+    the summary of this step window (profiles/<kernel>_<workload>-<steps>-steps_<kind>.md)
+    if there is one, else the workload's (profiles/<kernel>_<workload>_<kind>.md).
+    The density pass's work per step grows along a run, so the window matters."""
     for name in (f"{workload}-{steps}-steps", workload):
-        for kind in ("pmc", "trace"):
-            path = os.path.join(ROOT, "profiles", f"{kernel}_{name}_{kind}.md")
+        for kind in kinds:
+            rel = os.path.join("profiles", f"{kernel}_{name}_{kind}.md")
+            path = os.path.join(ROOT, rel)
             if not os.path.exists(path):
                 continue
             for line in open(path):
-                if line.startswith("traffic_bytes:"):
+                if line.startswith(key + ":"):
                     try:
-                        return float(line.split(":", 1)[1])
+                        return float(line.split(":", 1)[1]), rel
                     except ValueError:
-                        return None
-    return None
+                        return None, rel
+    return None, None
+
+
+def load_traffic(kernel: str, workload: str, steps: int):
+    """HBM bytes per launch of a kernel (FETCH_SIZE + WRITE_SIZE, calibrated) from
+    a committed --pmc or trace summary (line 'traffic_bytes: X'), if any."""
+    return _profile_value(kernel, workload, steps, ("pmc", "trace"), "traffic_bytes")[0]
+
+
+def load_valu_busy(kernel: str, workload: str, steps: int):
+    """Fraction of SIMD cycles issuing VALU instructions (SQ_INSTS_VALU x 4 over
+    1024 SIMDs x kernel cycles) from a committed SQ counter summary (line
+    'valu_busy: X'), and that summary's path, if any."""
+    return _profile_value(kernel, workload, steps, ("sq",), "valu_busy")
 
 
 def cpu_model() -> str:
@@ -324,6 +337,7 @@ def main():
     walk_kernels = KERNELS.get(args.walk, {})
     kernel = walk_kernels.get(dom, "k_" + dom)
     traffic = load_traffic(kernel, workload, steps)
+    valu_busy, valu_src = load_valu_busy(kernel, workload, steps)
     b_step = step_bytes(n_total, M_total, Ms_total)
     result = {
         "metric": METRIC,
@@ -366,6 +380,9 @@ def main():
             "avg_launch_us": avg_s * 1e6,
             "algorithmic_bytes_per_launch": sb[dom],
         },
+        # What does bound the dominant kernel when it is not HBM: its vector ALU
+        # issue, from the committed counter summary of this window (null if none).
+        "issue_bound": {"unit": "VALU issue", "busy": valu_busy, "source": valu_src},
         "step_roofline": {
             "b_alg_per_step": b_step,
             "achieved_GBs": b_step * steps / elapsed / 1e9,

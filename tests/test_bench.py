@@ -29,3 +29,12 @@ def test_traffic_of_the_20_step_window_comes_from_its_own_summary():
 
 def test_traffic_of_an_unprofiled_workload_is_null():
     assert bench.load_traffic("k_density_cw", "no-such-workload", 20) is None
+
+
+def test_valu_busy_of_the_window_and_of_the_run():
+    busy20, src20 = bench.load_valu_busy("k_density_cw", "dam-break-1m", 20)
+    busy, src = bench.load_valu_busy("k_density_cw", "dam-break-1m", 1000)
+    assert src20.endswith("k_density_cw_dam-break-1m-20-steps_sq.md")
+    assert src.endswith("k_density_cw_dam-break-1m_sq.md")
+    assert 0.0 < busy < busy20 <= 1.0
+    assert bench.load_valu_busy("k_density_cw", "no-such-workload", 20) == (None, None)
