@@ -646,7 +646,8 @@ constexpr bool kViscLists = SPH_CW_VISC_LISTS != 0;
 // Density pass, list mode with the viscosity sums: two candidates per batch (the
 // velocities take the registers of a third). One per batch fits 64 VGPRs and 8
 // waves per SIMD, but loses more to the shorter batches: 1M density 237 against
-// 226 us a step over 1000 steps.
+// 226 us a step over 1000 steps. Three per batch take 80 VGPRs with 2 spilled at 6
+// waves (232 us), or 82 at 5 waves (231 us).
 #ifndef SPH_CW_BATCH_DENSITY_LIST
 #define SPH_CW_BATCH_DENSITY_LIST 2
 #endif
